@@ -1,0 +1,149 @@
+/*
+ * soi.h -- public C ABI of the MI355X SOI FFT (drop-in for IntelLabs/SOI_FFT's soi.h).
+ *
+ * The descriptor create/compute/destroy API keeps the reference's names, argument meaning and
+ * caller-set fields (reference soi.h:236-280).  One process drives one GPU; the P MPI ranks of
+ * the reference become P virtual ranks on the device ("P ranks -> P virtual segments"): the
+ * caller sets N, P (and k), leaves rank = 0 and passes the whole N-element input.  On return
+ * the buffer holds all S = k*P segments of the N-point forward DFT in natural order, exactly
+ * the concatenation of what the reference's ranks 0..P-1 hold after compute_soi.
+ *
+ * Buffers may be device pointers (HBM-resident, the fast path) or host pointers (copied in and
+ * out around the GPU work); hipPointerGetAttributes tells them apart.
+ *
+ * Errors: the reference prints and exit()s (parallel_filter_subsampling.cpp:89-94, 537-548,
+ * pfft.c:157-183).  Here every entry point records a message, and soi_last_error() returns it;
+ * soi_last_status() returns its code (0 = success).  Nothing exits the process.
+ */
+#ifndef SOI_AMD_SOI_H_
+#define SOI_AMD_SOI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MPI is not needed on one GPU.  When the caller's translation unit has included mpi.h the
+ * real MPI_Comm is used; otherwise a placeholder integer type is declared.  The descriptor
+ * keeps the handle in an 8-byte slot either way, so the layout does not depend on it. */
+#ifndef MPI_COMM_WORLD
+typedef int MPI_Comm;
+#define SOI_NO_MPI_ 1
+#define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#endif
+
+typedef size_t cfft_size_t;                 /* reference soi.h:220 */
+
+typedef struct _complex_struct_t {          /* reference soi.h:222-225 */
+  double re;
+  double im;
+} complex_struct_t;
+
+/* reference soi.h:227: C99 double complex.  C++ callers see a layout-identical struct. */
+#ifdef __cplusplus
+typedef complex_struct_t cfft_complex_t;
+#else
+#include <complex.h>
+typedef double complex cfft_complex_t;
+#endif
+
+/* Precision / form selector (not in the reference, which is FP64 only).
+ *   SOI_PREC_FP64      plain FP64 path, the drop-in (default)
+ *   SOI_PREC_FP32      precision form: complex64 input and output, FP32 filter contraction
+ *                      and FFT arithmetic, FP64 windows, twiddles and demodulation */
+enum { SOI_PREC_FP64 = 0, SOI_PREC_FP32 = 1 };
+
+/* Status codes returned by soi_last_status(). */
+enum {
+  SOI_OK = 0,
+  SOI_ERR_ARGS = 1,          /* invalid or unsupported descriptor parameters */
+  SOI_ERR_TOO_SMALL = 2,     /* M/P < B (parallel_filter_subsampling.cpp:89-94) */
+  SOI_ERR_UNSUPPORTED_MU = 3,/* (n_mu,d_mu) not in {(5,4),(8,7)} (:537-548) */
+  SOI_ERR_ALLOC = 4,         /* device allocation failed (pfft.c:157-183) */
+  SOI_ERR_HIP = 5,           /* a HIP runtime call failed */
+  SOI_ERR_STATE = 6          /* descriptor used before init or after free */
+};
+
+/* Descriptor, reference soi.h:236-270.  Field names and meanings are kept; DFTI handles,
+ * w_dup and MPI requests are replaced by one opaque GPU plan. */
+typedef struct {
+  union { MPI_Comm comm; int64_t comm_slot_; };  /* unused on one GPU */
+  int P;                      /* number of (virtual) ranks */
+  int rank;                   /* must be 0 on one GPU */
+  cfft_size_t k;              /* segments per rank */
+  cfft_size_t N;              /* global vector length */
+  cfft_complex_t *W_inv;      /* device: inverse frequency window, M values */
+  cfft_complex_t *w;          /* device: time window, n_mu*B*S values, layout [theta][b][i] */
+  void *w_dup;                /* device: real filter taps used by the kernels */
+  cfft_complex_t *gamma_tilde;/* device: FFT work buffer, S*M_hat (caller may preallocate) */
+  cfft_complex_t *alpha_tilde;/* device: filtered and subsampled data [S][M_hat] (ditto) */
+  cfft_complex_t *beta_tilde; /* kept for layout compatibility; never allocated */
+  cfft_complex_t *alpha_ghost;/* unused: the halo is circular indexing in HBM */
+  int *delta, *epsilon;       /* device: VLC word streams (allocated when use_vlc) */
+
+  int n_mu;
+  int d_mu;
+  cfft_size_t B;
+  double tau;
+  double sigma;
+
+  void *plan;                 /* opaque GPU plan (replaces desc_dft_s / desc_dft_m_hat) */
+  void *sendRequests_, *recvRequests_;  /* layout slots for the reference's request arrays */
+  int use_vlc;                /* variable length compression of the exchanged data */
+  int *segmentBoundaries;     /* host, P+1 entries: first segment of each rank */
+  double comm_to_comp_cost_ratio;
+  int precision;              /* SOI_PREC_FP64 (default) or SOI_PREC_FP32 */
+} soi_desc_t;
+
+/* ---- reference API (soi.h:272-280) ---------------------------------------------------- */
+
+/* pfft.c:84-138: n_mu=5, d_mu=4, tau=928/1024, sigma=373.7314, B=72, use_vlc=0,
+ * comm_to_comp_cost_ratio=1; nulls the reusable buffers.  Also P=1, rank=0, k=1, N=0. */
+void set_default_soi_descriptor(soi_desc_t *desc);
+
+/* pfft.c:140-232: sets comm and k, segment boundaries p*k, allocates device buffers (unless
+ * preallocated), builds windows and taps on the GPU in FP64, plans the FFTs.  The caller must
+ * preset N and P (and rank = 0). */
+void init_soi_descriptor(soi_desc_t *desc, MPI_Comm comm, cfft_size_t k);
+
+/* pfft.c:276-721: in-place forward DFT of alpha_dt (N elements, device or host pointer).
+ * With precision == SOI_PREC_FP32 the buffer holds N complex64 values instead. */
+void compute_soi(soi_desc_t *d, cfft_complex_t *alpha_dt);
+
+/* pfft.c:234-267 */
+void free_soi_descriptor(soi_desc_t *d);
+
+/* parallel_filter_subsampling.cpp:534-550: Step 1 only.  Reads alpha_dt (N elements) and
+ * leaves d->alpha_tilde = [S][M_hat] (device). */
+void parallel_filter_subsampling(soi_desc_t *desc, cfft_complex_t *alpha_dt);
+
+/* ---- additions for a checked, stream-aware C ABI ----------------------------------------- */
+
+/* Message and code of the last error on this thread ("" and 0 after a successful call). */
+const char *soi_last_error(void);
+int soi_last_status(void);
+
+/* compute_soi on a caller-chosen HIP stream (hipStream_t passed as void*; NULL = the
+ * descriptor's own stream) without a final synchronisation: returns after enqueueing when
+ * alpha_dt is a device pointer. */
+void compute_soi_async(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream);
+
+/* Stream the descriptor's kernels run on (hipStream_t as void*). */
+void *soi_stream(soi_desc_t *d);
+
+/* Per-stage times in milliseconds of the last compute_soi (filter, vlc, fft pass 1,
+ * fft pass 2 + demodulation), measured with HIP events when timing was enabled. */
+void soi_enable_timing(soi_desc_t *d, int on);
+int soi_stage_times(soi_desc_t *d, double *ms, int max_stages);
+
+/* Per-segment maximum exponents and their global maximum from the last VLC run
+ * (pfft.c:319-343), and the per-segment send counts in doubles (pfft.c:404). */
+int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOI_AMD_SOI_H_ */

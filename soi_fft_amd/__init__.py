@@ -1,0 +1,211 @@
+"""soi_fft_amd -- SOI FFT for AMD Instinct MI355X behind the soi.h C ABI.
+
+The product is libsoi_hip.so (C host layer + HIP kernels, built by soi_fft_amd.build).  This
+module is a thin ctypes mirror of the C API (reference soi.h:236-288) so Python callers and the
+tests drive exactly the entry points a C caller would:
+
+    d = SoiDesc(); set_default_soi_descriptor(d)
+    d.N, d.P = 1 << 27, 32
+    init_soi_descriptor(d, k=1)
+    compute_soi(d, ptr)            # ptr: device (or host) address of N complex128
+    free_soi_descriptor(d)
+
+There is no fallback: if the library is missing or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsoi_hip.so")
+_lib = None
+
+SOI_PREC_FP64 = 0
+SOI_PREC_FP32 = 1
+
+
+class SoiError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"soi error {status}: {msg}")
+        self.status = status
+
+
+class SoiDesc(ctypes.Structure):
+    """Layout of soi_desc_t in include/soi.h (reference soi.h:236-270)."""
+    _fields_ = [
+        ("comm", ctypes.c_int64),
+        ("P", ctypes.c_int),
+        ("rank", ctypes.c_int),
+        ("k", ctypes.c_size_t),
+        ("N", ctypes.c_size_t),
+        ("W_inv", ctypes.c_void_p),
+        ("w", ctypes.c_void_p),
+        ("w_dup", ctypes.c_void_p),
+        ("gamma_tilde", ctypes.c_void_p),
+        ("alpha_tilde", ctypes.c_void_p),
+        ("beta_tilde", ctypes.c_void_p),
+        ("alpha_ghost", ctypes.c_void_p),
+        ("delta", ctypes.c_void_p),
+        ("epsilon", ctypes.c_void_p),
+        ("n_mu", ctypes.c_int),
+        ("d_mu", ctypes.c_int),
+        ("B", ctypes.c_size_t),
+        ("tau", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("plan", ctypes.c_void_p),
+        ("sendRequests_", ctypes.c_void_p),
+        ("recvRequests_", ctypes.c_void_p),
+        ("use_vlc", ctypes.c_int),
+        ("segmentBoundaries", ctypes.POINTER(ctypes.c_int)),
+        ("comm_to_comp_cost_ratio", ctypes.c_double),
+        ("precision", ctypes.c_int),
+    ]
+
+    # derived sizes (pfft.c:280-283)
+    @property
+    def S(self):
+        return self.k * self.P
+
+    @property
+    def M(self):
+        return self.N // self.S
+
+    @property
+    def M_hat(self):
+        return self.n_mu * self.M // self.d_mu
+
+
+def lib():
+    """Load libsoi_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with python -m soi_fft_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(SoiDesc)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.set_default_soi_descriptor.argtypes = [P]
+        L.init_soi_descriptor.argtypes = [P, ctypes.c_int, ctypes.c_size_t]
+        L.compute_soi.argtypes = [P, vp]
+        L.compute_soi_async.argtypes = [P, vp, vp]
+        L.free_soi_descriptor.argtypes = [P]
+        L.parallel_filter_subsampling.argtypes = [P, vp]
+        L.soi_last_error.restype = ctypes.c_char_p
+        L.soi_last_status.restype = ctypes.c_int
+        L.soi_stream.argtypes = [P]
+        L.soi_stream.restype = vp
+        L.soi_enable_timing.argtypes = [P, ctypes.c_int]
+        L.soi_stage_times.argtypes = [P, dp, ctypes.c_int]
+        L.soi_stage_times.restype = ctypes.c_int
+        L.soi_vlc_exponents.argtypes = [P, ip, ip, ip]
+        L.soi_vlc_exponents.restype = ctypes.c_int
+        L.max_exponent.argtypes = [vp, ctypes.c_int]
+        L.max_exponent.restype = ctypes.c_int
+        L.compress.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.compress.restype = ctypes.c_int
+        L.decompress.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.soi_max_exponent_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.soi_compress_batch.argtypes = [vp, ctypes.c_long, vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, vp, vp, vp]
+        L.soi_decompress_batch.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, vp, vp]
+        L.soi_fill_uniform.argtypes = [vp, ctypes.c_long, ctypes.c_uint64, vp]
+        L.soi_fill_uniform.restype = ctypes.c_int
+        L.soi_fill_tones.argtypes = [vp, ctypes.c_long, dp, dp, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_uint64, vp]
+        L.soi_fill_tones.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check():
+    L = lib()
+    st = L.soi_last_status()
+    if st:
+        raise SoiError(st, L.soi_last_error().decode())
+
+
+def set_default_soi_descriptor(d):
+    lib().set_default_soi_descriptor(ctypes.byref(d))
+
+
+def init_soi_descriptor(d, k=1, comm=0):
+    lib().init_soi_descriptor(ctypes.byref(d), comm, k)
+    _check()
+
+
+def compute_soi(d, ptr):
+    lib().compute_soi(ctypes.byref(d), ctypes.c_void_p(ptr))
+    _check()
+
+
+def compute_soi_async(d, ptr, stream=None):
+    lib().compute_soi_async(ctypes.byref(d), ctypes.c_void_p(ptr), ctypes.c_void_p(stream))
+    _check()
+
+
+def parallel_filter_subsampling(d, ptr):
+    lib().parallel_filter_subsampling(ctypes.byref(d), ctypes.c_void_p(ptr))
+    _check()
+
+
+def free_soi_descriptor(d):
+    lib().free_soi_descriptor(ctypes.byref(d))
+    _check()
+
+
+def stream_of(d):
+    return lib().soi_stream(ctypes.byref(d))
+
+
+def enable_timing(d, on=True):
+    lib().soi_enable_timing(ctypes.byref(d), int(on))
+
+
+def stage_times(d):
+    buf = (ctypes.c_double * 4)()
+    n = lib().soi_stage_times(ctypes.byref(d), buf, 4)
+    return list(buf[:n])
+
+
+def vlc_exponents(d):
+    S = d.S
+    seg = (ctypes.c_int * S)()
+    tot = ctypes.c_int()
+    cnt = (ctypes.c_int * S)()
+    n = lib().soi_vlc_exponents(ctypes.byref(d), seg, ctypes.byref(tot), cnt)
+    if n == 0:
+        return None
+    return list(seg), tot.value, list(cnt)
+
+
+def make_descriptor(N, P, k=1, n_mu=5, d_mu=4, B=72, tau=None, sigma=None, use_vlc=False,
+                    comm_to_comp_cost_ratio=1.0, precision=SOI_PREC_FP64):
+    """set_default_soi_descriptor + the caller-set fields (test.c:91-101, 125-128) + init."""
+    d = SoiDesc()
+    set_default_soi_descriptor(d)
+    d.N, d.P, d.rank = N, P, 0
+    d.n_mu, d.d_mu, d.B = n_mu, d_mu, B
+    if tau is not None:
+        d.tau = tau
+    if sigma is not None:
+        d.sigma = sigma
+    d.use_vlc = int(use_vlc)
+    d.comm_to_comp_cost_ratio = comm_to_comp_cost_ratio
+    d.precision = precision
+    init_soi_descriptor(d, k)
+    return d
+
+
+def soi_fft(x, P, k=1, **params):
+    """Convenience: in-place SOI FFT of a CUDA(HIP) complex128 torch tensor of length N.
+
+    Returns x (overwritten with the N-point forward DFT, natural order)."""
+    import torch  # plumbing only: device memory and synchronisation
+    assert x.is_cuda and x.dtype == torch.complex128 and x.is_contiguous()
+    d = make_descriptor(x.numel(), P, k, **params)
+    try:
+        torch.cuda.synchronize()
+        compute_soi(d, x.data_ptr())
+    finally:
+        free_soi_descriptor(d)
+    return x

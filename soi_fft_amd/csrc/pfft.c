@@ -1,0 +1,700 @@
+/*
+ * pfft.c -- C host layer of the MI355X SOI FFT: the soi.h descriptor API over the HIP kernels.
+ *
+ * Reference counterparts (IntelLabs/SOI_FFT):
+ *   set_default_soi_descriptor  pfft.c:84-138
+ *   init_soi_descriptor         pfft.c:140-232
+ *   free_soi_descriptor         pfft.c:234-267
+ *   compute_soi                 pfft.c:276-721
+ *   parallel_filter_subsampling parallel_filter_subsampling.cpp:534-550
+ * On one GPU the P ranks are P virtual ranks: the halo exchange becomes circular indexing in
+ * the filter kernel, the all-to-all becomes the segment-major layout of alpha_tilde, and the
+ * VLC branch compresses and decompresses every (segment, rank) chunk on the device.
+ */
+#define _GNU_SOURCE
+#include <limits.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/soi.h"
+#include "../../include/compress.h"
+#include "soi_kernels.h"
+
+/* ------------------------------------------------------------------------------------------ */
+/* error state                                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+static __thread char g_err[512];
+static __thread int g_status;
+
+static void set_err(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_status = code;
+}
+static void clear_err(void)
+{
+  g_err[0] = 0;
+  g_status = 0;
+}
+const char *soi_last_error(void) { return g_err; }
+int soi_last_status(void) { return g_status; }
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_err(SOI_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+              __FILE__, __LINE__);                                                     \
+      goto fail;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+#define KCHK(call)                                                                     \
+  do {                                                                                 \
+    int e_ = (call);                                                                   \
+    if (e_ != 0) {                                                                     \
+      set_err(SOI_ERR_HIP, "%s failed: %s (%s:%d)", #call,                             \
+              hipGetErrorString((hipError_t)e_), __FILE__, __LINE__);                  \
+      goto fail;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+/* ------------------------------------------------------------------------------------------ */
+/* plan                                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+#define NSTAGE 4
+typedef struct {
+  int device;
+  hipStream_t stream;
+  long N, S, M, Mh, l;
+  int P, k, n_mu, d_mu, B;
+  /* filter */
+  SoiFilterArgs fa;
+  double *taps;
+  soi_c64_t *post, *twS;
+  /* fft */
+  int fft_single, single_count;
+  int n1, n2, Tc, Tr, B1;
+  soi_c64_t *tw1, *tw2, *thi, *tlo;
+  /* buffers */
+  soi_c64_t *io;        /* staging for host-pointer inputs */
+  int owns_at;
+  /* vlc */
+  int *d_exps, *d_emi, *d_nwords;
+  int *h_exps, *seg_max, *send_counts, total_max;
+  int vlc_valid;
+  /* timing */
+  int timing;
+  hipEvent_t ev[NSTAGE + 1];
+  double stage_ms[NSTAGE];
+} soi_plan_t;
+
+static int is_smooth(long n)
+{
+  if (n <= 0) return 0;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+void set_default_soi_descriptor(soi_desc_t *desc)
+{
+  clear_err();
+  memset(desc, 0, sizeof(*desc));
+  desc->comm_slot_ = 0;
+  desc->P = 1;
+  desc->rank = 0;
+  desc->k = 1;
+  desc->N = 0;
+  desc->n_mu = 5;
+  desc->d_mu = 4;
+  desc->tau = 928 / 1024.;
+  desc->sigma = 373.7314;
+  desc->B = 72;
+  desc->use_vlc = 0;
+  desc->comm_to_comp_cost_ratio = 1;
+  desc->alpha_tilde = NULL;
+  desc->beta_tilde = NULL;
+  desc->gamma_tilde = NULL;
+  desc->precision = SOI_PREC_FP64;
+}
+
+/* Filter launch configuration: thread (lane, phase, tap half, J-stream); see soi_kernels.hip. */
+static int plan_filter(soi_plan_t *pl)
+{
+  SoiFilterArgs *a = &pl->fa;
+  memset(a, 0, sizeof(*a));
+  a->S = (int)pl->S;
+  a->n_mu = pl->n_mu;
+  a->d_mu = pl->d_mu;
+  a->NP = 2;
+  int half = (pl->B + 1) / 2;
+  static const int bps[] = {16, 24, 32, 36, 40};
+  a->BP = 0;
+  for (int t = 0; t < 5; ++t)
+    if (half <= bps[t]) {
+      a->BP = bps[t];
+      break;
+    }
+  if (!a->BP) {
+    set_err(SOI_ERR_ARGS, "B=%d is larger than the supported maximum of 80 taps", pl->B);
+    return -1;
+  }
+  int LB = (pl->n_mu == 5) ? 640 : 1024;
+  a->U = (pl->n_mu == 5) ? 16 : 8;
+  int base = a->S * pl->n_mu * a->NP;
+  a->W = 0;
+  for (int W = 1; W <= 64; ++W)
+    if ((base * W) % 64 == 0 && base * W <= LB) {
+      a->W = W;
+      break;
+    }
+  if (!a->W) {
+    set_err(SOI_ERR_ARGS, "S=%ld has no supported filter block shape (S*n_mu*2*W must be a "
+                          "multiple of 64 and at most %d)", pl->S, LB);
+    return -1;
+  }
+  int TJ = a->W * a->U;
+  int rows = TJ * pl->d_mu + a->NP * a->BP - pl->d_mu;
+  int grow = TJ * pl->n_mu;
+  long lds = (long)rows * a->S;
+  long g = (long)grow * (a->S + 1);
+  if (g > lds) lds = g;
+  lds *= 16;
+  if (lds > 160 * 1024) {
+    set_err(SOI_ERR_ARGS, "filter tile needs %ld bytes of LDS (> 160 KiB) at S=%ld", lds, pl->S);
+    return -1;
+  }
+  a->lds_bytes = (int)lds;
+  a->M = pl->M;
+  a->Mh = pl->Mh;
+  a->nJ = pl->M / pl->d_mu;
+  long ntiles = (a->nJ + TJ - 1) / TJ;
+  int per_cu = (int)((160 * 1024) / lds);
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 4) per_cu = 4;
+  long grid = 256L * per_cu;
+  if (grid > ntiles) grid = ntiles;
+  a->grid = (int)grid;
+  return 0;
+}
+
+/* largest T in {8,4,2,1} with n*T <= cap and T | other */
+static int pick_tile(int n, long other, int cap)
+{
+  static const int ts[] = {8, 4, 2, 1};
+  for (int t = 0; t < 4; ++t)
+    if ((long)n * ts[t] <= cap && other % ts[t] == 0) return ts[t];
+  return 0;
+}
+
+static int plan_fft(soi_plan_t *pl)
+{
+  const long Mh = pl->Mh;
+  const int CAP = 5120; /* complex values per block: 80 KiB of LDS, two blocks per CU */
+  if (!is_smooth(Mh)) {
+    set_err(SOI_ERR_ARGS, "M_hat=%ld is not of the form 2^a 3^b 5^c", Mh);
+    return -1;
+  }
+  if (Mh + 1 <= CAP) {
+    pl->fft_single = 1;
+    pl->single_count = (int)(CAP / (Mh + 1));
+    if (pl->single_count > pl->S) pl->single_count = (int)pl->S;
+    if (pl->single_count < 1) pl->single_count = 1;
+    return 0;
+  }
+  pl->fft_single = 0;
+  /* n1 x n2 = Mh, both smooth, column tile Tc >= 2 and row tile Tr >= 2 preferred; minimise
+   * max(n1, n2) with n1 a power of two when possible */
+  long best_cost = LONG_MAX;
+  int bn1 = 0, bn2 = 0, bTc = 0, bTr = 0;
+  for (long n1 = 2; n1 <= CAP; ++n1) {
+    if (Mh % n1) continue;
+    long n2 = Mh / n1;
+    if (n2 > CAP || !is_smooth(n1) || !is_smooth(n2)) continue;
+    int Tc = pick_tile((int)n1, n2, CAP), Tr = pick_tile((int)n2, n1, CAP);
+    if (!Tc || !Tr) continue;
+    long cost = (n1 > n2 ? n1 : n2) * 4 + (Tc < 2) * 100000 + (Tr < 2) * 100000 +
+                ((n1 & (n1 - 1)) ? 1 : 0);
+    if (cost < best_cost) {
+      best_cost = cost;
+      bn1 = (int)n1;
+      bn2 = (int)n2;
+      bTc = Tc;
+      bTr = Tr;
+    }
+  }
+  if (!bn1) {
+    set_err(SOI_ERR_ARGS, "M_hat=%ld is too long for the two-pass segment FFT (max %d^2)", Mh,
+            CAP);
+    return -1;
+  }
+  pl->n1 = bn1;
+  pl->n2 = bn2;
+  pl->Tc = bTc;
+  pl->Tr = bTr;
+  int b1 = 1;
+  while ((long)b1 * b1 < Mh) b1 <<= 1;
+  pl->B1 = b1;
+  return 0;
+}
+
+static void free_plan(soi_plan_t *pl)
+{
+  if (!pl) return;
+  if (pl->device >= 0) (void)hipSetDevice(pl->device);
+  (void)hipFree(pl->taps);
+  (void)hipFree(pl->post);
+  (void)hipFree(pl->twS);
+  (void)hipFree(pl->tw1);
+  (void)hipFree(pl->tw2);
+  (void)hipFree(pl->thi);
+  (void)hipFree(pl->tlo);
+  (void)hipFree(pl->io);
+  (void)hipFree(pl->d_exps);
+  (void)hipFree(pl->d_emi);
+  (void)hipFree(pl->d_nwords);
+  free(pl->h_exps);
+  free(pl->seg_max);
+  free(pl->send_counts);
+  for (int e = 0; e <= NSTAGE; ++e)
+    if (pl->ev[e]) (void)hipEventDestroy(pl->ev[e]);
+  if (pl->stream) (void)hipStreamDestroy(pl->stream);
+  free(pl);
+}
+
+void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
+{
+  clear_err();
+  soi_plan_t *pl = NULL;
+  d->comm = comm;
+  d->k = k;
+  if (d->plan) {
+    set_err(SOI_ERR_STATE, "descriptor already initialised; call free_soi_descriptor first");
+    return;
+  }
+  if (d->P < 1 || k < 1 || k > 1024 || d->N == 0) {
+    set_err(SOI_ERR_ARGS, "invalid N=%zu, P=%d, k=%zu (N>0, P>=1, 1<=k<=1024)", d->N, d->P, k);
+    return;
+  }
+  if (d->rank != 0) {
+    set_err(SOI_ERR_ARGS, "rank must be 0: one GPU runs all P ranks as virtual ranks");
+    return;
+  }
+  if (!((d->n_mu == 5 && d->d_mu == 4) || (d->n_mu == 8 && d->d_mu == 7))) {
+    set_err(SOI_ERR_UNSUPPORTED_MU,
+            "Unsupported n_mu and d_mu (%d/%d). Try n_mu=5 && d_mu=4 or n_mu=8 && d_mu=7",
+            d->n_mu, d->d_mu);
+    return;
+  }
+  if (d->precision != SOI_PREC_FP64) {
+    set_err(SOI_ERR_ARGS, "precision %d is not available in this build", d->precision);
+    return;
+  }
+  long S = (long)k * d->P;
+  if (d->N % S) {
+    set_err(SOI_ERR_ARGS, "S=k*P=%ld must divide N=%zu", S, d->N);
+    return;
+  }
+  long M = (long)d->N / S;
+  if (S % 4 || !is_smooth(S)) {
+    set_err(SOI_ERR_ARGS, "S=%ld must be a multiple of 4 (pfft.c:192) of the form 2^a 3^b 5^c", S);
+    return;
+  }
+  if (M % ((long)d->P * d->d_mu)) {
+    set_err(SOI_ERR_ARGS, "P*d_mu=%ld must divide M=N/S=%ld", (long)d->P * d->d_mu, M);
+    return;
+  }
+  if (M / d->P < (long)d->B) {
+    set_err(SOI_ERR_TOO_SMALL, "input size too small: M/P=%ld < B=%zu", M / d->P, d->B);
+    return;
+  }
+  if (d->B < 2 || d->B <= (cfft_size_t)d->d_mu) {
+    set_err(SOI_ERR_ARGS, "B=%zu must exceed d_mu=%d", d->B, d->d_mu);
+    return;
+  }
+
+  pl = (soi_plan_t *)calloc(1, sizeof(soi_plan_t));
+  if (!pl) {
+    set_err(SOI_ERR_ALLOC, "host allocation failed");
+    return;
+  }
+  pl->device = -1;
+  HIPCHK(hipGetDevice(&pl->device));
+  pl->N = (long)d->N;
+  pl->P = d->P;
+  pl->k = (int)k;
+  pl->S = S;
+  pl->M = M;
+  pl->n_mu = d->n_mu;
+  pl->d_mu = d->d_mu;
+  pl->B = (int)d->B;
+  pl->Mh = (long)d->n_mu * M / d->d_mu;
+  pl->l = pl->Mh / d->P;
+  if (plan_filter(pl) || plan_fft(pl)) goto fail_quiet;
+  HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+
+  /* segment boundaries p*k (pfft.c:144-147) */
+  d->segmentBoundaries = (int *)malloc(sizeof(int) * (d->P + 1));
+  for (int p = 0; p <= d->P; ++p) d->segmentBoundaries[p] = p * (int)k;
+
+  {
+    int B_pad = pl->fa.NP * pl->fa.BP;
+    size_t ntap = (size_t)d->n_mu * S * B_pad;
+    HIPCHK(hipMalloc((void **)&pl->taps, ntap * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&pl->post, (size_t)d->n_mu * S * sizeof(soi_c64_t)));
+    HIPCHK(hipMalloc((void **)&d->w, (size_t)d->n_mu * d->B * S * sizeof(soi_c64_t)));
+    HIPCHK(hipMalloc((void **)&d->W_inv, (size_t)M * sizeof(soi_c64_t)));
+    HIPCHK(hipMalloc((void **)&pl->twS, (size_t)S * sizeof(soi_c64_t)));
+    d->w_dup = pl->taps;
+    if (!d->alpha_tilde) {
+      hipError_t e = hipMalloc((void **)&d->alpha_tilde, (size_t)S * pl->Mh * sizeof(soi_c64_t));
+      if (e != hipSuccess) {
+        d->alpha_tilde = NULL;
+        set_err(SOI_ERR_ALLOC, "Failed to allocate d->alpha_tilde (%zu bytes): %s",
+                (size_t)S * pl->Mh * sizeof(soi_c64_t), hipGetErrorString(e));
+        goto fail_quiet;
+      }
+      pl->owns_at = 1;
+    }
+    KCHK(soik_make_tables(pl->taps, pl->post, (soi_c64_t *)d->w, (soi_c64_t *)d->W_inv, (int)S,
+                          d->n_mu, d->d_mu, (int)d->B, B_pad, d->tau, d->sigma, M, pl->stream));
+    KCHK(soik_make_twiddles(pl->twS, S, S, 1, pl->stream));
+    pl->fa.taps = pl->taps;
+    pl->fa.post = pl->post;
+    pl->fa.twS = pl->twS;
+    if (pl->fft_single) {
+      HIPCHK(hipMalloc((void **)&pl->tw1, (size_t)pl->Mh * sizeof(soi_c64_t)));
+      KCHK(soik_make_twiddles(pl->tw1, pl->Mh, pl->Mh, 1, pl->stream));
+    } else {
+      long nhi = (pl->Mh + pl->B1 - 1) / pl->B1;
+      HIPCHK(hipMalloc((void **)&pl->tw1, (size_t)pl->n1 * sizeof(soi_c64_t)));
+      HIPCHK(hipMalloc((void **)&pl->tw2, (size_t)pl->n2 * sizeof(soi_c64_t)));
+      HIPCHK(hipMalloc((void **)&pl->thi, (size_t)nhi * sizeof(soi_c64_t)));
+      HIPCHK(hipMalloc((void **)&pl->tlo, (size_t)pl->B1 * sizeof(soi_c64_t)));
+      KCHK(soik_make_twiddles(pl->tw1, pl->n1, pl->n1, 1, pl->stream));
+      KCHK(soik_make_twiddles(pl->tw2, pl->n2, pl->n2, 1, pl->stream));
+      KCHK(soik_make_twiddles(pl->thi, nhi, pl->Mh, pl->B1, pl->stream));
+      KCHK(soik_make_twiddles(pl->tlo, pl->B1, pl->Mh, 1, pl->stream));
+    }
+    if (d->use_vlc) {
+      long nch = S * d->P;
+      HIPCHK(hipMalloc((void **)&d->delta, (size_t)nch * 4 * pl->l * sizeof(int) + 256));
+      HIPCHK(hipMalloc((void **)&pl->d_exps, nch * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&pl->d_emi, nch * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&pl->d_nwords, nch * sizeof(int)));
+      pl->h_exps = (int *)malloc(nch * sizeof(int));
+      pl->seg_max = (int *)malloc(S * sizeof(int));
+      pl->send_counts = (int *)malloc(S * sizeof(int));
+    }
+    for (int e = 0; e <= NSTAGE; ++e) HIPCHK(hipEventCreate(&pl->ev[e]));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+  }
+  d->plan = pl;
+  return;
+fail:
+fail_quiet:
+  if (pl) {
+    if (pl->owns_at && d->alpha_tilde) (void)hipFree(d->alpha_tilde), d->alpha_tilde = NULL;
+    (void)hipFree(d->w);
+    (void)hipFree(d->W_inv);
+    (void)hipFree(d->delta);
+    d->w = NULL;
+    d->W_inv = NULL;
+    d->delta = NULL;
+    d->w_dup = NULL;
+    free(d->segmentBoundaries);
+    d->segmentBoundaries = NULL;
+    free_plan(pl);
+  }
+}
+
+void free_soi_descriptor(soi_desc_t *d)
+{
+  clear_err();
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised");
+    return;
+  }
+  (void)hipSetDevice(pl->device);
+  (void)hipStreamSynchronize(pl->stream);
+  (void)hipFree(d->w);
+  (void)hipFree(d->W_inv);
+  (void)hipFree(d->delta);
+  if (pl->owns_at) (void)hipFree(d->alpha_tilde);
+  d->alpha_tilde = NULL;
+  d->beta_tilde = NULL;
+  d->gamma_tilde = NULL;
+  d->w = NULL;
+  d->W_inv = NULL;
+  d->w_dup = NULL;
+  d->delta = NULL;
+  d->epsilon = NULL;
+  free(d->segmentBoundaries);
+  d->segmentBoundaries = NULL;
+  free_plan(pl);
+  d->plan = NULL;
+}
+
+/* device pointer?  (hipPointerGetAttributes fails or says host for plain host memory) */
+static int is_device_ptr(const void *p)
+{
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+static void mark(soi_plan_t *pl, int e)
+{
+  if (pl->timing) (void)hipEventRecord(pl->ev[e], pl->stream);
+}
+
+/* pfft.c:390-440 restated: per-segment send counts and cost-balanced segment boundaries */
+static void vlc_plan(soi_desc_t *d, soi_plan_t *pl)
+{
+  int S = (int)pl->S, P = pl->P, k = pl->k;
+  long l = pl->l, Mh = pl->Mh;
+  int totalMax = pl->total_max;
+  double *pcs = (double *)malloc(sizeof(double) * (S + 1));
+  pcs[0] = 0;
+  for (int p = 0; p < P; ++p)
+    for (int ik = 0; ik < k; ++ik) {
+      int s = p * k + ik;
+      int bitsPerInput = 52 + 3 - (totalMax - pl->seg_max[s]);
+      if (bitsPerInput > 31) ++bitsPerInput;
+      int lenPerSimdLane = (int)((l * 2 / 16 * bitsPerInput + 31) / 32);
+      int remainder = (int)((l * 2 % 16 * bitsPerInput + 31) / 32);
+      int sendLen = lenPerSimdLane * 16 + remainder;
+      if (bitsPerInput <= 0) sendLen = 0;
+      pl->send_counts[s] = (sendLen + 1) / 2;
+      pcs[s + 1] = pcs[s] + d->comm_to_comp_cost_ratio * pl->send_counts[s] / 2 / Mh + 1;
+    }
+  double costPerRank = pcs[S] / P;
+  int s = 1;
+  for (int p = 1; p < P;) {
+    if (pcs[s] >= p * costPerRank) {
+      if (pcs[s] - p * costPerRank > p * costPerRank - pcs[s - 1])
+        d->segmentBoundaries[p] = s - 1;
+      else
+        d->segmentBoundaries[p] = s;
+      ++p;
+    } else {
+      ++s;
+    }
+  }
+  free(pcs);
+}
+
+static int run_filter(soi_desc_t *d, soi_plan_t *pl, const soi_c64_t *x, hipStream_t st)
+{
+  pl->fa.x = x;
+  pl->fa.at = d->alpha_tilde;
+  int e = soik_filter_sdft(&pl->fa, st);
+  if (e) {
+    set_err(SOI_ERR_HIP, "filter kernel launch failed: %s", hipGetErrorString((hipError_t)e));
+    return -1;
+  }
+  return 0;
+}
+
+static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
+{
+  const int S = (int)pl->S, P = pl->P;
+  const long l = pl->l, Mh = pl->Mh;
+  const int nch = S * P;
+  double *at = (double *)d->alpha_tilde;
+  /* chunk (s, p) = alpha_tilde[s][p*l .. (p+1)*l): 2l doubles at at + 2*(s*Mh + p*l);
+   * with Mh = P*l the chunks are contiguous in (s, p) order, stride 2l doubles */
+  KCHK(soik_max_exponent(at, (int)(2 * l), nch, pl->d_exps, st));
+  HIPCHK(hipMemcpyAsync(pl->h_exps, pl->d_exps, nch * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  /* Allreduce(MAX) over ranks (pfft.c:331-333), then the global maximum (:340-343) */
+  pl->total_max = INT_MIN;
+  for (int s = 0; s < S; ++s) {
+    int m = INT_MIN;
+    for (int p = 0; p < P; ++p)
+      if (pl->h_exps[s * P + p] > m) m = pl->h_exps[s * P + p];
+    pl->seg_max[s] = m;
+    if (m > pl->total_max) pl->total_max = m;
+  }
+  vlc_plan(d, pl);
+  for (int s = 0; s < S; ++s)
+    for (int p = 0; p < P; ++p) pl->h_exps[s * P + p] = pl->seg_max[s];
+  HIPCHK(hipMemcpyAsync(pl->d_emi, pl->h_exps, nch * sizeof(int), hipMemcpyHostToDevice, st));
+  KCHK(soik_compress(d->delta, 4 * l, at, (int)(2 * l), nch, pl->total_max, pl->d_emi,
+                     pl->d_nwords, st));
+  KCHK(soik_decompress(at, d->delta, 4 * l, (int)(2 * l), nch, pl->total_max, pl->d_emi, st));
+  pl->vlc_valid = 1;
+  return 0;
+fail:
+  return -1;
+}
+
+static int run_fft(soi_desc_t *d, soi_plan_t *pl, soi_c64_t *X, hipStream_t st)
+{
+  soi_c64_t *at = (soi_c64_t *)d->alpha_tilde;
+  if (pl->fft_single) {
+    mark(pl, 2);
+    KCHK(soik_fft_single(at, X, pl->tw1, (const soi_c64_t *)d->W_inv, pl->Mh, pl->M, (int)pl->S,
+                         pl->single_count, st));
+    mark(pl, 3);
+  } else {
+    KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
+                      (int)pl->S, st));
+    mark(pl, 3);
+    KCHK(soik_fft_row(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Tr,
+                      pl->Mh, pl->M, (int)pl->S, st));
+  }
+  return 0;
+fail:
+  return -1;
+}
+
+static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, int sync,
+                         int filter_only)
+{
+  clear_err();
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised (call init_soi_descriptor)");
+    return;
+  }
+  if (!alpha_dt) {
+    set_err(SOI_ERR_ARGS, "alpha_dt is NULL");
+    return;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : pl->stream;
+  HIPCHK(hipSetDevice(pl->device));
+  {
+    int dev = is_device_ptr(alpha_dt);
+    soi_c64_t *buf = (soi_c64_t *)alpha_dt;
+    size_t bytes = (size_t)pl->N * sizeof(soi_c64_t);
+    if (!dev) {
+      if (!pl->io) {
+        hipError_t e = hipMalloc((void **)&pl->io, bytes);
+        if (e != hipSuccess) {
+          pl->io = NULL;
+          set_err(SOI_ERR_ALLOC, "staging allocation of %zu bytes failed", bytes);
+          return;
+        }
+      }
+      HIPCHK(hipMemcpyAsync(pl->io, alpha_dt, bytes, hipMemcpyHostToDevice, st));
+      buf = pl->io;
+    }
+    pl->vlc_valid = 0;
+    mark(pl, 0);
+    if (run_filter(d, pl, buf, st)) return;
+    mark(pl, 1);
+    if (!filter_only) {
+      if (d->use_vlc && run_vlc(d, pl, st)) return;
+      if (!pl->fft_single) mark(pl, 2);
+      if (run_fft(d, pl, buf, st)) return;
+      mark(pl, 4);
+    }
+    if (!dev) {
+      if (!filter_only)
+        HIPCHK(hipMemcpyAsync(alpha_dt, pl->io, bytes, hipMemcpyDeviceToHost, st));
+      sync = 1;
+    }
+    if (sync) HIPCHK(hipStreamSynchronize(st));
+    if (pl->timing && sync) {
+      for (int e = 0; e < NSTAGE; ++e) pl->stage_ms[e] = 0;
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]) == hipSuccess) pl->stage_ms[0] = ms;
+      if (!filter_only) {
+        if (hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]) == hipSuccess) pl->stage_ms[1] = ms;
+        if (hipEventElapsedTime(&ms, pl->ev[2], pl->ev[3]) == hipSuccess) pl->stage_ms[2] = ms;
+        if (hipEventElapsedTime(&ms, pl->ev[3], pl->ev[4]) == hipSuccess) pl->stage_ms[3] = ms;
+      }
+    }
+  }
+  return;
+fail:
+  return;
+}
+
+void compute_soi(soi_desc_t *d, cfft_complex_t *alpha_dt) { compute_impl(d, alpha_dt, NULL, 1, 0); }
+
+void compute_soi_async(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream)
+{
+  compute_impl(d, alpha_dt, stream, 0, 0);
+}
+
+void parallel_filter_subsampling(soi_desc_t *desc, cfft_complex_t *alpha_dt)
+{
+  compute_impl(desc, alpha_dt, NULL, 1, 1);
+}
+
+void *soi_stream(soi_desc_t *d)
+{
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  return pl ? (void *)pl->stream : NULL;
+}
+
+void soi_enable_timing(soi_desc_t *d, int on)
+{
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  if (pl) pl->timing = on;
+}
+
+int soi_stage_times(soi_desc_t *d, double *ms, int max_stages)
+{
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  if (!pl) return 0;
+  int n = max_stages < NSTAGE ? max_stages : NSTAGE;
+  for (int e = 0; e < n; ++e) ms[e] = pl->stage_ms[e];
+  return n;
+}
+
+int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts)
+{
+  soi_plan_t *pl = (soi_plan_t *)d->plan;
+  if (!pl || !pl->vlc_valid) return 0;
+  if (seg_max_exp) memcpy(seg_max_exp, pl->seg_max, sizeof(int) * pl->S);
+  if (total_max_exp) *total_max_exp = pl->total_max;
+  if (send_counts) memcpy(send_counts, pl->send_counts, sizeof(int) * pl->S);
+  return (int)pl->S;
+}
+
+/* Device-resident helpers for tests and the benchmark: seeded synthetic inputs. */
+int soi_fill_uniform(void *x, long N, uint64_t seed, void *stream)
+{
+  int e = soik_gen_uniform((soi_c64_t *)x, N, seed, (hipStream_t)stream);
+  if (!e) e = (int)hipStreamSynchronize((hipStream_t)stream);
+  return e;
+}
+
+int soi_fill_tones(void *x, long N, const double *freqs, const double *phases, int ntones,
+                   double amp, double noise_sigma, uint64_t seed, void *stream)
+{
+  double *df = NULL, *dp = NULL;
+  int e = 0;
+  if (ntones > 0) {
+    if ((e = (int)hipMalloc((void **)&df, ntones * sizeof(double)))) return e;
+    if ((e = (int)hipMalloc((void **)&dp, ntones * sizeof(double)))) {
+      (void)hipFree(df);
+      return e;
+    }
+    (void)hipMemcpy(df, freqs, ntones * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(dp, phases, ntones * sizeof(double), hipMemcpyHostToDevice);
+  }
+  e = soik_gen_tones((soi_c64_t *)x, N, df, dp, ntones, amp, noise_sigma, seed,
+                     (hipStream_t)stream);
+  if (!e) e = (int)hipStreamSynchronize((hipStream_t)stream);
+  (void)hipFree(df);
+  (void)hipFree(dp);
+  return e;
+}

@@ -1,0 +1,61 @@
+/*
+ * soi_kernels.h -- internal C ABI between the C host layer (pfft.c, compress_host.c) and the
+ * HIP kernels (soi_kernels.hip).  Device pointers only; every launcher enqueues on `st` and
+ * returns a hipError_t value (0 = success).
+ */
+#ifndef SOI_AMD_KERNELS_H_
+#define SOI_AMD_KERNELS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <hip/hip_runtime.h>
+typedef double2 soi_c64_t;
+typedef float2 soi_c32_t;
+extern "C" {
+#else
+#include <hip/hip_runtime_api.h>
+typedef struct { double x, y; } soi_c64_t;
+typedef struct { float x, y; } soi_c32_t;
+#endif
+
+typedef struct SoiFilterArgs {
+  const void *x;      /* input [M][S], complex64/128 */
+  void *at;           /* output alpha_tilde [S][M_hat] */
+  const double *taps; /* [n_mu][S][NP*BP] */
+  const soi_c64_t *post;
+  const void *twS;    /* S-point twiddles */
+  long M, Mh, nJ;
+  int S, n_mu, d_mu, BP, NP, U, W;
+  int grid;
+  int lds_bytes;
+} SoiFilterArgs;
+
+int soik_make_tables(double *taps, soi_c64_t *post, soi_c64_t *w_full, soi_c64_t *winv, int S,
+                     int n_mu, int d_mu, int B, int B_pad, double tau, double sigma, long M,
+                     hipStream_t st);
+int soik_make_twiddles(soi_c64_t *tw, long n, long L, long mult, hipStream_t st);
+int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st);
+int soik_fft_single(const soi_c64_t *at, soi_c64_t *X, const soi_c64_t *tw,
+                    const soi_c64_t *winv, long L, long M, int S, int count, hipStream_t st);
+int soik_fft_col(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
+                 const soi_c64_t *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
+                 hipStream_t st);
+int soik_fft_row(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
+                 const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
+                 hipStream_t st);
+int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st);
+int soik_compress(int *out, long stride, const double *x, int len, int nchunk, int e_max,
+                  const int *emi, int *nwords, hipStream_t st);
+int soik_decompress(double *out, const int *in, long stride, int len, int nchunk, int e_max,
+                    const int *emi, hipStream_t st);
+int soik_gen_uniform(soi_c64_t *x, long N, uint64_t seed, hipStream_t st);
+int soik_gen_tones(soi_c64_t *x, long N, const double *tf, const double *tph, int ntones,
+                   double amp, double noise_sigma, uint64_t seed, hipStream_t st);
+int soik_f64_to_f32(const soi_c64_t *in, soi_c32_t *out, long n, hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,709 @@
+// soi_kernels.hip -- MI355X (gfx950) kernels of the SOI FFT hot path and their C-ABI launchers.
+//
+// Data layout in HBM (one GPU, P virtual ranks, S = k*P segments):
+//   input  alpha[q][i]            q < M rows, i < S lanes (the natural-order N-vector)
+//   alpha_tilde[s][R]             s < S segments, R < M_hat (segment-major: the reference's
+//                                 all-to-all, pfft.c:553-578, is the identity in this layout)
+//   output X[s*M + m]             written over alpha (in place, as the reference does)
+//
+// Stages (reference -> here):
+//   Step 1  parallel_filter_subsampling.cpp:134-524  -> k_filter_sdft   (conv + S-point DFT)
+//   Step 2  exchange pfft.c:474-583                  -> identity (layout) / VLC codec kernels
+//   Step 3  pfft.c:642 DFTI M_hat-point              -> k_fft_single or k_fft_col + k_fft_row
+//           pfft.c:649-693 demodulation              -> fused into the last FFT pass's store
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "soi_common.hpp"
+#include "soi_kernels.h"
+
+using namespace soi;
+
+// =============================================================================================
+// Tables (init only; FP64 on the GPU)
+// =============================================================================================
+
+// Real filter taps and complex post factors.  The reference window (pfft.c:42-66) is
+//   w[theta][b][i] = e^{i pi t} y(t) / mu,  t = theta*d/n - b - i/S + kappa/2,
+//   y(t) = sin(pi tau t)/(pi tau t) * exp(-pi^2 t^2 / sigma), y(0) = 1.
+// Since e^{i pi t} = e^{i pi (t+b)} (-1)^b, w = post[theta][i] * taps[theta][i][b] with
+//   taps = (-1)^b y(t)/mu  (real)  and  post = e^{i pi (theta d/n + kappa/2 - i/S)}.
+// t and the phase are formed from exact integer numerators over 2*n*S.
+__global__ void k_make_taps(double *taps, double2 *post, double2 *w_full, int S, int n_mu,
+                            int d_mu, int B, int B_pad, double tau, double sigma)
+{
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long total = (long)n_mu * S * B_pad;
+  if (idx >= total) return;
+  int b = (int)(idx % B_pad);
+  long rest = idx / B_pad;
+  int i = (int)(rest % S);
+  int theta = (int)(rest / S);
+  const long den = 2L * n_mu * S;
+  const int kappa = B - d_mu;
+  long num0 = 2L * theta * d_mu * S + (long)kappa * n_mu * S - 2L * i * n_mu;  // t + b, scaled
+  double tap = 0.0;
+  if (b < B) {
+    long num = num0 - 2L * b * n_mu * S;
+    double t = (double)num / (double)den;
+    double y;
+    if (num == 0) {
+      y = 1.0;
+    } else {
+      double x = tau * t;
+      y = sinpi(x) / (M_PI * x) * exp(-M_PI * M_PI * t * t / sigma);
+    }
+    tap = ((b & 1) ? -y : y) * ((double)d_mu / (double)n_mu);
+  }
+  taps[idx] = tap;
+  // phase e^{i pi num0/den}: reduce num0 modulo 2*den (period 2 in units of pi)
+  long r = num0 % (2 * den);
+  if (r < 0) r += 2 * den;
+  double sp, cp;
+  sincospi((double)r / (double)den, &sp, &cp);
+  if (b == 0) post[theta * S + i] = double2{cp, sp};
+  // full complex window in the reference's logical layout [theta][b][i] (descriptor field w)
+  if (w_full && b < B) w_full[((long)theta * B + b) * S + i] = double2{cp * tap, sp * tap};
+}
+
+// W_inv[m] = e^{-2 pi i kappa m / (2M)} / w_hat(m/M - 1/2)   (pfft.c:68-82)
+// w_hat(t) = [erfc(sqrt(sigma)(t - tau/2)) - erfc(sqrt(sigma)(t + tau/2))] / (2 tau)
+__global__ void k_make_winv(double2 *winv, long M, int kappa, double tau, double sigma)
+{
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  double t = (double)m / (double)M - 0.5;
+  double ss = sqrt(sigma);
+  double y = 1.0 / (2.0 * tau) * (erfc(ss * (t - tau / 2)) - erfc(ss * (t + tau / 2)));
+  long r = ((long)kappa * m) % (2 * M);  // angle = -pi * r / M
+  double sp, cp;
+  sincospi((double)r / (double)M, &sp, &cp);
+  winv[m] = double2{cp / y, -sp / y};
+}
+
+// tw[m] = e^{-2 pi i m / L} for m < n (exact reduction: m/L in [0,1))
+__global__ void k_make_twiddles(double2 *tw, long n, long L, long mult)
+{
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  long e = (m * mult) % L;
+  double sp, cp;
+  sincospi(2.0 * (double)e / (double)L, &sp, &cp);
+  tw[m] = double2{cp, -sp};
+}
+
+__global__ void k_f64_to_f32(const double2 *in, float2 *out, long n)
+{
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m < n) out[m] = float2{(float)in[m].x, (float)in[m].y};
+}
+
+// =============================================================================================
+// Step 1: filter + subsample + S-point DFT (FP64)
+// =============================================================================================
+// Work tile: TJ = W*U consecutive row groups J (output rows R = J*n_mu + theta) for all S lanes.
+// Thread (i, theta, p, w) owns lane i, phase theta, tap half p (taps b in [p*BP, (p+1)*BP)) and
+// J-stream w (U consecutive J).  Its BP taps stay in registers for the whole kernel; the input
+// column comes from an LDS tile of TJ*d + B_pad - d rows x S lanes, each LDS value feeding up
+// to U accumulators.  The two tap halves are summed through LDS, multiplied by post[theta][i]
+// and laid out as G[R][i]; the S-point DFT of every row is done in LDS and the tile is stored
+// segment-major, TJ*n_mu contiguous values per segment.
+// Fully unrolled polyphase accumulation: input row o of the thread's column feeds accumulator u
+// with tap b = o - u*d whenever 0 <= b < BP.  Expanded at compile time so that every tap and
+// accumulator index is a constant (taps and accumulators stay in registers).
+template <int O, int UU, int U, int DMU, int BP>
+__device__ __forceinline__ void conv_taps(const double2 xv, const double (&h)[BP], double2 (&acc)[U])
+{
+  if constexpr (UU < U) {
+    constexpr int b = O - UU * DMU;
+    if constexpr (b >= 0 && b < BP) {
+      acc[UU].x = fma(h[b], xv.x, acc[UU].x);
+      acc[UU].y = fma(h[b], xv.y, acc[UU].y);
+    }
+    conv_taps<O, UU + 1, U, DMU, BP>(xv, h, acc);
+  }
+}
+
+template <int O, int OEND, int U, int DMU, int BP>
+__device__ __forceinline__ void conv_rows(const double2 *__restrict__ col, const int S,
+                                          const double (&h)[BP], double2 (&acc)[U])
+{
+  if constexpr (O < OEND) {
+    const double2 xv = col[O * S];
+    conv_taps<O, 0, U, DMU, BP>(xv, h, acc);
+    conv_rows<O + 1, OEND, U, DMU, BP>(col, S, h, acc);
+  }
+}
+
+template <int NMU, int DMU, int BP, int U, int NP, int LB, int MINW>
+__global__ __launch_bounds__(LB, MINW) void k_filter_sdft_f64(
+    const double2 *__restrict__ x, double2 *__restrict__ at, const double *__restrict__ taps,
+    const double2 *__restrict__ post, const double2 *__restrict__ twS, long M,
+    int S, long Mh, int W, long nJ)
+{
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int i = tid % S;
+  int rest = tid / S;
+  const int theta = rest % NMU;
+  rest /= NMU;
+  const int p = rest % NP;
+  const int w = rest / NP;
+  const int TJ = W * U;
+  const int rows = TJ * DMU + NP * BP - DMU;  // input rows of one tile
+  const int grow = TJ * NMU;                  // output rows of one tile
+  const int gs = S + 1;                       // padded row stride of the G tile
+
+  double h[BP];
+  {
+    const double *tp = taps + ((long)theta * S + i) * (NP * BP) + p * BP;
+#pragma unroll
+    for (int b = 0; b < BP; ++b) h[b] = tp[b];
+  }
+  const double2 pf = post[theta * S + i];
+  const long ntiles = (nJ + TJ - 1) / TJ;
+
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long J0 = tile * TJ;
+    const long q0 = J0 * DMU;
+    // stage the input rows (circular: the last rank's halo is rank 0's first rows)
+    for (int e = tid; e < rows * S; e += nt) {
+      int r = e / S, ln = e - r * S;
+      long q = q0 + r;
+      if (q >= M) q -= M;
+      lds[e] = x[q * S + ln];
+    }
+    __syncthreads();
+
+    double2 acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = double2{0.0, 0.0};
+    const double2 *col = lds + (w * U * DMU + p * BP) * S + i;
+    conv_rows<0, (U - 1) * DMU + BP, U, DMU, BP>(col, S, h, acc);
+    __syncthreads();  // the input tile is dead from here on; G reuses its space
+    double2 *G = lds;
+    if (NP == 2 && p == 1) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) G[((w * U + u) * NMU + theta) * gs + i] = acc[u];
+    }
+    if (NP == 2) __syncthreads();
+    if (p == 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double2 a = acc[u];
+        if (NP == 2) a = cadd(a, G[((w * U + u) * NMU + theta) * gs + i]);
+        G[((w * U + u) * NMU + theta) * gs + i] = cmul(a, pf);
+      }
+    }
+    __syncthreads();
+    // S-point DFT of each row (in place; sequence stride gs, element stride 1)
+    lds_fft<10>(G, S, grow, gs, 1, twS);
+    // segment-major store: at[s][J0*n_mu + r]
+    const long R0 = J0 * NMU;
+    for (int e = tid; e < grow * S; e += nt) {
+      int s = e / grow, r = e - s * grow;
+      if (R0 + r < Mh) at[(long)s * Mh + R0 + r] = G[r * gs + s];
+    }
+    __syncthreads();
+  }
+}
+
+// =============================================================================================
+// Step 3: M_hat-point FFT per segment, demodulation fused into the last pass
+// =============================================================================================
+// Single pass: `count` whole segments per block (M_hat small enough for LDS).
+__global__ __launch_bounds__(256) void k_fft_single_f64(const double2 *__restrict__ at,
+                                                         double2 *__restrict__ X,
+                                                         const double2 *__restrict__ tw,
+                                                         const double2 *__restrict__ winv,
+                                                         long L, long M, int S,
+                                                         int count)
+{
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int s0 = blockIdx.x * count;
+  const int nseq = min(count, S - s0);
+  const int qs = (int)L + 1;
+  for (int e = tid; e < nseq * L; e += nt) {
+    int q = e / (int)L, j = e - q * (int)L;
+    lds[q * qs + j] = at[(long)(s0 + q) * L + j];
+  }
+  __syncthreads();
+  lds_fft<32>(lds, (int)L, nseq, qs, 1, tw);
+  for (int e = tid; e < nseq * M; e += nt) {
+    int q = e / (int)M, m = e - q * (int)M;
+    X[(long)(s0 + q) * M + m] = cmul(winv[m], lds[q * qs + m]);
+  }
+}
+
+// Pass 1 (column FFTs): segment viewed as n1 x n2 row-major (R = n2*r1 + r2).  A block takes
+// Tc adjacent columns, runs n1-point FFTs down them, multiplies element m1 of column r2 by
+// e^{-2 pi i r2 m1 / M_hat} (two-level table: hi[e / B1] * lo[e % B1]) and stores in place.
+__global__ __launch_bounds__(256) void k_fft_col_f64(double2 *__restrict__ data,
+                                                      const double2 *__restrict__ tw1,
+                                                      const double2 *__restrict__ thi,
+                                                      const double2 *__restrict__ tlo, int n1, int n2, int Tc, int B1, long Mh,
+                                                      int S)
+{
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ncb = n2 / Tc;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = bid % ncb;
+  const int s = bid / ncb;
+  if (s >= S) return;
+  const int c0 = cb * Tc;
+  double2 *seg = data + (long)s * Mh;
+  for (int e = tid; e < n1 * Tc; e += nt) {
+    int r1 = e / Tc, q = e - r1 * Tc;
+    lds[e] = seg[(long)r1 * n2 + c0 + q];
+  }
+  __syncthreads();
+  lds_fft<32>(lds, n1, Tc, 1, Tc, tw1);
+  for (int e = tid; e < n1 * Tc; e += nt) {
+    int m1 = e / Tc, q = e - m1 * Tc;
+    long ex = (long)(c0 + q) * m1;  // < M_hat
+    double2 t = cmul(thi[ex / B1], tlo[ex % B1]);
+    seg[(long)m1 * n2 + c0 + q] = cmul(lds[e], t);
+  }
+}
+
+// Pass 2 (row FFTs + demodulation): a block takes Tr adjacent rows m1, runs n2-point FFTs along
+// them; element m2 of row m1 is y[m1 + n1*m2].  For m < M it stores W_inv[m] * y[m] to X[s*M+m]
+// (pfft.c:649-693), Tr adjacent m per contiguous run.
+__global__ __launch_bounds__(256) void k_fft_row_f64(const double2 *__restrict__ data,
+                                                      double2 *__restrict__ X,
+                                                      const double2 *__restrict__ tw2,
+                                                      const double2 *__restrict__ winv,
+                                                      int n1, int n2, int Tr, long Mh,
+                                                      long M, int S)
+{
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nrb = n1 / Tr;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = bid % S;          // segments of one row block are adjacent: W_inv rows shared
+  const int rb = bid / S;
+  if (rb >= nrb) return;
+  const int a = rb * Tr;
+  const double2 *seg = data + (long)s * Mh;
+  for (int e = tid; e < n2 * Tr; e += nt) {
+    int q = e / n2, r2 = e - q * n2;
+    lds[r2 * Tr + q] = seg[(long)(a + q) * n2 + r2];
+  }
+  __syncthreads();
+  lds_fft<32>(lds, n2, Tr, 1, Tr, tw2);
+  double2 *out = X + (long)s * M;
+  for (int e = tid; e < n2 * Tr; e += nt) {
+    int m2 = e / Tr, q = e - m2 * Tr;
+    long m = (long)(a + q) + (long)n1 * m2;
+    if (m < M) out[m] = cmul(winv[m], lds[e]);
+  }
+}
+
+// =============================================================================================
+// VLC codec (compress.c).  Every 16-element chunk of a stream contributes one field per lane;
+// field widths are fixed per stream, so each 32-bit word of lane j is computable directly.
+// =============================================================================================
+__device__ __forceinline__ int exp_field(double v)
+{
+  uint64_t u = (uint64_t)__double_as_longlong(v);
+  return (int)((u >> 52) & 0x7FF);
+}
+
+// per-chunk max exponent (compress.c:46-93).  One block per chunk.
+__global__ void k_max_exponent(const double *__restrict__ x, int len, int *__restrict__ out)
+{
+  __shared__ int red[1024];
+  const double *c = x + (long)blockIdx.x * len;
+  int m = INT_MIN;
+  for (int e = threadIdx.x; e < len; e += blockDim.x) m = max(m, exp_field(c[e]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = (int)((unsigned)red[0] - 1023u);
+}
+
+struct VlcParams {
+  int i1Len, i2Len, fbits;  // field widths; fbits = bits per element in a lane
+  double e1, e2, e3;        // 2^(21-e_max), 2^(52-e_max), 2^31
+  double d1, d2;            // 2^(e_max-21), 2^(e_max-52)
+};
+
+__device__ __forceinline__ VlcParams vlc_params(int e_max, int e_max_i)
+{
+  VlcParams v;
+  int dl = e_max - e_max_i;
+  v.i1Len = 24 - dl;
+  v.i2Len = min(32, 55 - dl);
+  v.fbits = (v.i1Len > 0 ? v.i1Len : 0) + (v.i2Len > 0 ? v.i2Len : 0);
+  v.e1 = ldexp(1.0, 21 - e_max);
+  v.e2 = ldexp(1.0, 52 - e_max);
+  v.e3 = ldexp(1.0, 31);
+  v.d1 = ldexp(1.0, e_max - 21);
+  v.d2 = ldexp(1.0, e_max - 52);
+  return v;
+}
+
+// The bit pattern of one element: i1Len bits of i1 (when i1Len > 0) followed by i2Len bits of
+// i2, as a value of fbits <= 56 bits, MSB first.  vec=1: rint + fmsub (compress.c:230-280);
+// vec=0: round() + unfused (compress.c:333-342).
+__device__ __forceinline__ uint64_t vlc_encode(double x, const VlcParams &v, bool vec)
+{
+  double d1, d2;
+  if (vec) {
+    d1 = rint(x * v.e1);
+    d2 = rint(fma(x, v.e2, -(d1 * v.e3)));
+  } else {
+    d1 = round(x * v.e1);
+    double pr = __dmul_rn(x, v.e2);
+    d2 = round(__dsub_rn(pr, __dmul_rn(d1, v.e3)));
+  }
+  uint32_t i1 = (uint32_t)(int32_t)d1, i2 = (uint32_t)(int32_t)d2;
+  uint64_t f2 = v.i2Len >= 32 ? (uint64_t)i2 : ((uint64_t)i2 & ((1ull << v.i2Len) - 1));
+  if (v.i1Len > 0) {
+    uint64_t f1 = (uint64_t)i1 & ((1ull << v.i1Len) - 1);
+    return (f1 << v.i2Len) | f2;
+  }
+  return f2;
+}
+
+// bits [pos, pos+32) of a lane stream made of consecutive fbits-wide fields (MSB first).
+template <typename F>
+__device__ __forceinline__ uint32_t vlc_word(long pos, int fbits, long nfields, F field)
+{
+  uint32_t word = 0;
+  long f0 = pos / fbits;
+  long f1 = (pos + 31) / fbits;
+  for (long f = f0; f <= f1 && f < nfields; ++f) {
+    uint64_t val = field(f);
+    long fstart = f * fbits;
+    // overlap of [fstart, fstart+fbits) with [pos, pos+32)
+    long lo = max(fstart, pos), hi = min(fstart + fbits, pos + 32);
+    if (lo >= hi) continue;
+    int nb = (int)(hi - lo);
+    int shift_out = (int)(fstart + fbits - hi);  // bits of the field after hi
+    uint64_t part = (val >> shift_out) & ((1ull << nb) - 1);
+    word |= (uint32_t)(part << (pos + 32 - hi));
+  }
+  return word;
+}
+
+// compress: one thread per output word.  Stream c: vector rows (16 lanes interleaved) then the
+// scalar tail.  nwords[c] = compress()'s return value.
+__global__ void k_compress(int *__restrict__ out, long stride, const double *__restrict__ x,
+                           int len, int nchunk, int e_max, const int *__restrict__ emi,
+                           int *__restrict__ nwords)
+{
+  const int c = blockIdx.y;
+  if (c >= nchunk) return;
+  VlcParams v = vlc_params(e_max, emi[c]);
+  const double *in = x + (long)c * len;
+  int *o = out + (long)c * stride;
+  if (v.i2Len <= 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) nwords[c] = 0;
+    return;
+  }
+  const long nvf = len / 16;                 // fields per lane in the vector body
+  const long vrows = (nvf * v.fbits + 31) / 32;
+  const long ntail = len - nvf * 16;
+  const long twords = (ntail * v.fbits + 31) / 32;
+  const long total = vrows * 16 + twords;
+  if (blockIdx.x == 0 && threadIdx.x == 0) nwords[c] = (int)total;
+  for (long wi = blockIdx.x * (long)blockDim.x + threadIdx.x; wi < total;
+       wi += (long)gridDim.x * blockDim.x) {
+    uint32_t word;
+    if (wi < vrows * 16) {
+      long row = wi / 16;
+      int lane = (int)(wi % 16);
+      word = vlc_word(row * 32, v.fbits, nvf,
+                      [&](long f) { return vlc_encode(in[f * 16 + lane], v, true); });
+    } else {
+      long tw = wi - vrows * 16;
+      const double *tin = in + nvf * 16;
+      word = vlc_word(tw * 32, v.fbits, ntail, [&](long f) { return vlc_encode(tin[f], v, false); });
+    }
+    o[wi] = (int)word;
+  }
+}
+
+// extract the fbits-wide field f of a lane stream whose word w is at base[w*wstride]
+__device__ __forceinline__ uint64_t vlc_field(const int *base, long wstride, long f, int fbits)
+{
+  long pos = f * fbits;
+  long w0 = pos / 32;
+  int off = (int)(pos - w0 * 32);
+  uint64_t hi = (uint32_t)base[w0 * wstride];
+  uint64_t lo = (off + fbits > 32) ? (uint32_t)base[(w0 + 1) * wstride] : 0u;
+  uint64_t bits = (hi << 32) | lo;  // 64-bit window starting at word w0
+  int sh = 64 - off - fbits;
+  if (off + fbits > 64) {
+    // field spans three words (fbits <= 56 and off <= 31 can reach 87 bits)
+    uint64_t lo2 = (uint32_t)base[(w0 + 2) * wstride];
+    int extra = off + fbits - 64;
+    uint64_t top = bits & ((fbits - extra >= 64) ? ~0ull : ((1ull << (64 - off)) - 1));
+    return (top << extra) | (lo2 >> (32 - extra));
+  }
+  return (bits >> sh) & ((fbits >= 64) ? ~0ull : ((1ull << fbits) - 1));
+}
+
+__device__ __forceinline__ double vlc_decode(uint64_t fv, const VlcParams &v, bool vec)
+{
+  int32_t i1 = 0, i2;
+  uint64_t f2 = v.i2Len >= 32 ? (fv & 0xFFFFFFFFull) : (fv & ((1ull << v.i2Len) - 1));
+  // sign-extend i2Len bits
+  i2 = (int32_t)((int64_t)(f2 << (64 - v.i2Len)) >> (64 - v.i2Len));
+  if (v.i1Len > 0) {
+    uint64_t f1 = (fv >> v.i2Len) & ((1ull << v.i1Len) - 1);
+    i1 = (int32_t)((int64_t)(f1 << (64 - v.i1Len)) >> (64 - v.i1Len));
+  }
+  if (vec) return fma((double)i1, v.d1, (double)i2 * v.d2);       // compress.c:472-483
+  return __dadd_rn(__dmul_rn((double)i1, v.d1), __dmul_rn((double)i2, v.d2));  // :529-531
+}
+
+__global__ void k_decompress(double *__restrict__ out, const int *__restrict__ in, long stride,
+                             int len, int nchunk, int e_max, const int *__restrict__ emi)
+{
+  const int c = blockIdx.y;
+  if (c >= nchunk) return;
+  VlcParams v = vlc_params(e_max, emi[c]);
+  double *o = out + (long)c * len;
+  const int *w = in + (long)c * stride;
+  const long nvf = len / 16;
+  const long vrows = (nvf * v.fbits + 31) / 32;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < len;
+       e += (long)gridDim.x * blockDim.x) {
+    if (v.i2Len <= 0) {
+      o[e] = 0.0;
+      continue;
+    }
+    double val;
+    if (e < nvf * 16) {
+      long f = e / 16;
+      int lane = (int)(e % 16);
+      val = vlc_decode(vlc_field(w + lane, 16, f, v.fbits), v, true);
+    } else {
+      long f = e - nvf * 16;
+      val = vlc_decode(vlc_field(w + vrows * 16, 1, f, v.fbits), v, false);
+    }
+    o[e] = val;
+  }
+}
+
+// =============================================================================================
+// Synthetic inputs (seeded, counter based) for tests and the benchmark
+// =============================================================================================
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t n)
+{
+  return (double)(splitmix64(seed ^ (n * 0xD1B54A32D192ED03ull)) >> 11) * 0x1.0p-53;
+}
+
+// uniform[-1,1] re/im (identical to the oracle's orc_uniform_input)
+__global__ void k_gen_uniform(double2 *x, long N, uint64_t seed)
+{
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  x[n] = double2{2.0 * u01(seed, 2 * n) - 1.0, 2.0 * u01(seed, 2 * n + 1) - 1.0};
+}
+
+// sum of tones a_j e^{2 pi i f_j n / N + i phi_j} plus complex Gaussian noise (Box-Muller)
+__global__ void k_gen_tones(double2 *x, long N, const double *__restrict__ tf,
+                            const double *__restrict__ tph, int ntones, double amp,
+                            double noise_sigma, uint64_t seed)
+{
+  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double re = 0, im = 0;
+  for (int j = 0; j < ntones; ++j) {
+    // phase in turns: f*n/N exactly reduced using the integer part of f
+    double f = tf[j];
+    double fi = floor(f), ff = f - fi;
+    long fn = ((long)fi % N) * (n % N) % N;  // (fi*n) mod N, exact for N < 2^31
+    double turns = (double)fn / (double)N + ff * ((double)n / (double)N) + tph[j];
+    turns -= floor(turns);
+    double sp, cp;
+    sincospi(2.0 * turns, &sp, &cp);
+    re += amp * cp;
+    im += amp * sp;
+  }
+  if (noise_sigma > 0) {
+    double u1 = u01(seed, 2 * n), u2 = u01(seed, 2 * n + 1);
+    double rr = sqrt(-2.0 * log(1.0 - u1)) * noise_sigma;
+    double sp, cp;
+    sincospi(2.0 * u2, &sp, &cp);
+    re += rr * cp;
+    im += rr * sp;
+  }
+  x[n] = double2{re, im};
+}
+
+// =============================================================================================
+// C-ABI launchers
+// =============================================================================================
+static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+extern "C" int soik_make_tables(double *taps, double2 *post, double2 *w_full, double2 *winv,
+                                int S, int n_mu, int d_mu, int B, int B_pad, double tau,
+                                double sigma, long M, hipStream_t st)
+{
+  long nt = (long)n_mu * S * B_pad;
+  hipLaunchKernelGGL(k_make_taps, dim3(nblk(nt, 256)), dim3(256), 0, st, taps, post, w_full, S,
+                     n_mu, d_mu, B, B_pad, tau, sigma);
+  hipLaunchKernelGGL(k_make_winv, dim3(nblk(M, 256)), dim3(256), 0, st, winv, M, B - d_mu, tau,
+                     sigma);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_make_twiddles(double2 *tw, long n, long L, long mult, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_make_twiddles, dim3(nblk(n, 256)), dim3(256), 0, st, tw, n, L, mult);
+  return (int)hipGetLastError();
+}
+
+// Filter kernel configurations: (n_mu, d_mu) in {(5,4), (8,7)}, BP in {16,24,32,36,40}.
+// (5,4) blocks have up to 640 threads and are register-capped for 3 waves per SIMD;
+// (8,7) blocks have up to 1024 threads (S = 64 at n_mu = 8).
+template <int NMU, int DMU, int BP, int U>
+static int launch_filter(const SoiFilterArgs *a, hipStream_t st)
+{
+  constexpr int LB = (NMU == 5) ? 640 : 1024;
+  constexpr int MINW = (NMU == 5) ? 3 : 4;
+  const int threads = a->S * NMU * a->NP * a->W;
+  if (threads > LB || threads % 64) return (int)hipErrorInvalidValue;
+  auto k2 = k_filter_sdft_f64<NMU, DMU, BP, U, 2, LB, MINW>;
+  auto k1 = k_filter_sdft_f64<NMU, DMU, BP, U, 1, LB, MINW>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)k2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr_set = true;
+  }
+  auto kern = (a->NP == 2) ? k2 : k1;
+  hipLaunchKernelGGL(kern, dim3(a->grid), dim3(threads), a->lds_bytes, st, (const double2 *)a->x,
+                     (double2 *)a->at, a->taps, a->post, (const double2 *)a->twS, a->M, a->S,
+                     a->Mh, a->W, a->nJ);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
+{
+#define SOI_BP_CASES(NMU, DMU, U)                                  \
+  switch (a->BP) {                                                 \
+    case 16: return launch_filter<NMU, DMU, 16, U>(a, st);         \
+    case 24: return launch_filter<NMU, DMU, 24, U>(a, st);         \
+    case 32: return launch_filter<NMU, DMU, 32, U>(a, st);         \
+    case 36: return launch_filter<NMU, DMU, 36, U>(a, st);         \
+    case 40: return launch_filter<NMU, DMU, 40, U>(a, st);         \
+    default: return (int)hipErrorInvalidValue;                     \
+  }
+  if (a->n_mu == 5 && a->d_mu == 4 && a->U == 16) { SOI_BP_CASES(5, 4, 16) }
+  if (a->n_mu == 8 && a->d_mu == 7 && a->U == 8) { SOI_BP_CASES(8, 7, 8) }
+#undef SOI_BP_CASES
+  return (int)hipErrorInvalidValue;
+}
+
+static void set_lds_attr(const void *f)
+{
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+extern "C" int soik_fft_single(const double2 *at, double2 *X, const double2 *tw,
+                               const double2 *winv, long L, long M, int S, int count,
+                               hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_single_f64); once = true; }
+  size_t lds = (size_t)count * (L + 1) * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_single_f64, dim3((S + count - 1) / count), dim3(256), lds, st, at, X,
+                     tw, winv, L, M, S, count);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_fft_col(double2 *data, const double2 *tw1, const double2 *thi,
+                            const double2 *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
+                            hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_col_f64); once = true; }
+  size_t lds = (size_t)n1 * Tc * sizeof(double2);
+  unsigned grid = (unsigned)((n2 / Tc) * S);
+  hipLaunchKernelGGL(k_fft_col_f64, dim3(grid), dim3(256), lds, st, data, tw1, thi, tlo, n1, n2,
+                     Tc, B1, Mh, S);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_fft_row(const double2 *data, double2 *X, const double2 *tw2,
+                            const double2 *winv, int n1, int n2, int Tr, long Mh, long M, int S,
+                            hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_row_f64); once = true; }
+  size_t lds = (size_t)n2 * Tr * sizeof(double2);
+  unsigned grid = (unsigned)((n1 / Tr) * S);
+  hipLaunchKernelGGL(k_fft_row_f64, dim3(grid), dim3(256), lds, st, data, X, tw2, winv, n1, n2, Tr,
+                     Mh, M, S);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st)
+{
+  if (nchunk <= 0) return 0;
+  hipLaunchKernelGGL(k_max_exponent, dim3(nchunk), dim3(256), 0, st, x, len, out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_compress(int *out, long stride, const double *x, int len, int nchunk,
+                             int e_max, const int *emi, int *nwords, hipStream_t st)
+{
+  if (nchunk <= 0) return 0;
+  long maxw = (long)len * 56 / 32 / 1 + 64;
+  unsigned gx = nblk(maxw, 256);
+  if (gx > 4096) gx = 4096;
+  hipLaunchKernelGGL(k_compress, dim3(gx, nchunk), dim3(256), 0, st, out, stride, x, len, nchunk,
+                     e_max, emi, nwords);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_decompress(double *out, const int *in, long stride, int len, int nchunk,
+                               int e_max, const int *emi, hipStream_t st)
+{
+  if (nchunk <= 0) return 0;
+  unsigned gx = nblk(len, 256);
+  if (gx > 4096) gx = 4096;
+  hipLaunchKernelGGL(k_decompress, dim3(gx, nchunk), dim3(256), 0, st, out, in, stride, len,
+                     nchunk, e_max, emi);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_gen_uniform(double2 *x, long N, uint64_t seed, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_gen_uniform, dim3(nblk(N, 256)), dim3(256), 0, st, x, N, seed);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_gen_tones(double2 *x, long N, const double *tf, const double *tph, int ntones,
+                              double amp, double noise_sigma, uint64_t seed, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_gen_tones, dim3(nblk(N, 256)), dim3(256), 0, st, x, N, tf, tph, ntones,
+                     amp, noise_sigma, seed);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_f64_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n, 256)), dim3(256), 0, st, in, out, n);
+  return (int)hipGetLastError();
+}

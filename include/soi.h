@@ -55,6 +55,14 @@ typedef double complex cfft_complex_t;
  *                      and FFT arithmetic, FP64 windows, twiddles and demodulation */
 enum { SOI_PREC_FP64 = 0, SOI_PREC_FP32 = 1 };
 
+/* Window evaluation (not in the reference).
+ *   SOI_WINDOW_AUTO       exact windows with real filter taps where they agree with the
+ *                         reference's window values to far below 1e-12 in the output
+ *                         ((n_mu,d_mu) = (5,4) and B >= 60), the reference's values otherwise
+ *   SOI_WINDOW_REFERENCE  the reference's window values: t formed in double as pfft.c:52 does
+ *   SOI_WINDOW_EXACT      t from its exact rational value (more accurate than the reference) */
+enum { SOI_WINDOW_AUTO = 0, SOI_WINDOW_REFERENCE = 1, SOI_WINDOW_EXACT = 2 };
+
 /* Status codes returned by soi_last_status(). */
 enum {
   SOI_OK = 0,
@@ -95,6 +103,7 @@ typedef struct {
   int *segmentBoundaries;     /* host, P+1 entries: first segment of each rank */
   double comm_to_comp_cost_ratio;
   int precision;              /* SOI_PREC_FP64 (default) or SOI_PREC_FP32 */
+  int window_mode;            /* SOI_WINDOW_AUTO (default), _REFERENCE or _EXACT */
 } soi_desc_t;
 
 /* ---- reference API (soi.h:272-280) ---------------------------------------------------- */

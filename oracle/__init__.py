@@ -60,6 +60,7 @@ def lib():
                                       ctypes.c_double, ip, ip, ip]
         _lib.orc_uniform_input.argtypes = [ctypes.c_long, ctypes.c_uint64, dp]
         _lib.orc_num_threads.restype = ctypes.c_int
+        _lib.orc_set_window_mode.argtypes = [ctypes.c_int]
     return _lib
 
 
@@ -87,6 +88,11 @@ def dims(p):
 
 def as_f64(c):
     return np.ascontiguousarray(np.asarray(c, dtype=np.complex128)).view(np.float64)
+
+
+def set_window_mode(mode):
+    """0: windows exactly as the reference forms t (default); 1: exact rational t."""
+    lib().orc_set_window_mode(mode)
 
 
 def w_table(p):

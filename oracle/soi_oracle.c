@@ -59,6 +59,12 @@ static const double PI_D = 3.14159265358979323846;
  * reference forms it, the trigonometric parts are evaluated in long double.
  * Layout: w[(theta*B + b)*S + i] (the logical index theta*B*S + j*S + ii of pfft.c:196).
  * ---------------------------------------------------------------------------------------- */
+/* 0: t formed in double as the reference does (pfft.c:52); 1: t = exact rational rounded once;
+ * 2: reference t in the real window y(t), exact t in the phase e^{i pi t}.  Modes 1 and 2 exist
+ * to measure how much of the error budget the reference's rounding of t takes, and where. */
+static int g_window_mode = 0;
+void orc_set_window_mode(int mode) { g_window_mode = mode; }
+
 static void w_f(const orc_params *p, long idx, double *re, double *im)
 {
   long S = ORC_S(p), M = ORC_M(p), Mh = ORC_MHAT(p);
@@ -66,6 +72,12 @@ static void w_f(const orc_params *p, long idx, double *re, double *im)
   long kappa = p->B - p->d_mu;
   long j = idx % ((long)p->B * S);
   double t = ((double)theta / Mh - (double)j / p->N + (double)kappa / (2 * M)) * M;
+  /* t = theta*d/n - j/S + kappa/2 = (2 theta d S - 2 j n + kappa n S) / (2 n S) */
+  long tnum = 2 * theta * p->d_mu * S - 2 * j * p->n_mu + kappa * p->n_mu * S;
+  double t_exact = (double)((long double)tnum / (long double)(2 * p->n_mu * S));
+  double t_phase = t;
+  if (g_window_mode == 1) t = t_phase = t_exact;
+  if (g_window_mode == 2) t_phase = t_exact;
   long double y;
   if (t == 0) {
     y = 1;
@@ -73,7 +85,7 @@ static void w_f(const orc_params *p, long idx, double *re, double *im)
     y = sinl(PI_L * p->tau * t) / (PI_D * p->tau * t) * exp(-PI_D * PI_D * t * t / p->sigma);
   }
   double yd = (double)y;
-  double rr = (double)cosl(PI_L * t), ri = (double)sinl(PI_L * t);
+  double rr = (double)cosl(PI_L * t_phase), ri = (double)sinl(PI_L * t_phase);
   double mu = (double)p->n_mu / p->d_mu;
   /* r*y/mu in double complex arithmetic (pfft.c:62-65) */
   *re = rr * yd / mu;

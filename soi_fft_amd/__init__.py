@@ -21,6 +21,9 @@ _lib = None
 
 SOI_PREC_FP64 = 0
 SOI_PREC_FP32 = 1
+SOI_WINDOW_AUTO = 0
+SOI_WINDOW_REFERENCE = 1
+SOI_WINDOW_EXACT = 2
 
 
 class SoiError(RuntimeError):
@@ -58,6 +61,7 @@ class SoiDesc(ctypes.Structure):
         ("segmentBoundaries", ctypes.POINTER(ctypes.c_int)),
         ("comm_to_comp_cost_ratio", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("window_mode", ctypes.c_int),
     ]
 
     # derived sizes (pfft.c:280-283)
@@ -80,6 +84,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is missing: build it with python -m soi_fft_amd.build")
+        # A process must hold one HIP runtime.  PyTorch ships its own libamdhip64 (same soname,
+        # different file); when torch is present it is loaded first so that libsoi_hip.so binds
+        # to that runtime instead of loading a second one from /opt/rocm.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER(SoiDesc)
         vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
@@ -179,7 +190,8 @@ def vlc_exponents(d):
 
 
 def make_descriptor(N, P, k=1, n_mu=5, d_mu=4, B=72, tau=None, sigma=None, use_vlc=False,
-                    comm_to_comp_cost_ratio=1.0, precision=SOI_PREC_FP64):
+                    comm_to_comp_cost_ratio=1.0, precision=SOI_PREC_FP64,
+                    window_mode=SOI_WINDOW_AUTO):
     """set_default_soi_descriptor + the caller-set fields (test.c:91-101, 125-128) + init."""
     d = SoiDesc()
     set_default_soi_descriptor(d)
@@ -192,6 +204,7 @@ def make_descriptor(N, P, k=1, n_mu=5, d_mu=4, B=72, tau=None, sigma=None, use_v
     d.use_vlc = int(use_vlc)
     d.comm_to_comp_cost_ratio = comm_to_comp_cost_ratio
     d.precision = precision
+    d.window_mode = window_mode
     init_soi_descriptor(d, k)
     return d
 

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libsoi_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("SOI_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["soi_kernels.hip"]
+HIP_SOURCES = ["soi_kernels.hip", "soi_filter.hip"]
 C_SOURCES = ["pfft.c", "compress_host.c"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.

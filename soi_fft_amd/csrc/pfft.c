@@ -77,7 +77,8 @@ typedef struct {
   int P, k, n_mu, d_mu, B;
   /* filter */
   SoiFilterArgs fa;
-  double *taps;
+  int ref_windows;
+  void *taps;
   soi_c64_t *post, *twS;
   /* fft */
   int fft_single, single_count;
@@ -125,65 +126,47 @@ void set_default_soi_descriptor(soi_desc_t *desc)
   desc->beta_tilde = NULL;
   desc->gamma_tilde = NULL;
   desc->precision = SOI_PREC_FP64;
+  desc->window_mode = SOI_WINDOW_AUTO;
 }
 
-/* Filter launch configuration: thread (lane, phase, tap half, J-stream); see soi_kernels.hip. */
-static int plan_filter(soi_plan_t *pl)
+/* Filter launch configuration (see soi_filter.hip).  Window evaluation: exact windows with real
+ * taps for (5,4) and B >= 60, where they agree with the reference's window values to ~1e-14 in
+ * the output; otherwise the reference's own window values as complex taps. */
+static int plan_filter(soi_plan_t *pl, int window_mode)
 {
   SoiFilterArgs *a = &pl->fa;
   memset(a, 0, sizeof(*a));
   a->S = (int)pl->S;
   a->n_mu = pl->n_mu;
   a->d_mu = pl->d_mu;
-  a->NP = 2;
-  int half = (pl->B + 1) / 2;
-  static const int bps[] = {16, 24, 32, 36, 40};
-  a->BP = 0;
-  for (int t = 0; t < 5; ++t)
-    if (half <= bps[t]) {
-      a->BP = bps[t];
-      break;
-    }
-  if (!a->BP) {
-    set_err(SOI_ERR_ARGS, "B=%d is larger than the supported maximum of 80 taps", pl->B);
+  int rt_ok = (pl->n_mu == 5 && pl->d_mu == 4 && pl->B >= 60);
+  if (window_mode == SOI_WINDOW_AUTO) {
+    a->ct = !rt_ok;
+    pl->ref_windows = !rt_ok;
+  } else if (window_mode == SOI_WINDOW_REFERENCE) {
+    a->ct = 1;
+    pl->ref_windows = 1;
+  } else if (window_mode == SOI_WINDOW_EXACT) {
+    a->ct = !rt_ok;
+    pl->ref_windows = 0;
+  } else {
+    set_err(SOI_ERR_ARGS, "unknown window_mode %d", window_mode);
     return -1;
   }
-  int LB = (pl->n_mu == 5) ? 640 : 1024;
-  a->U = (pl->n_mu == 5) ? 16 : 8;
-  int base = a->S * pl->n_mu * a->NP;
-  a->W = 0;
-  for (int W = 1; W <= 64; ++W)
-    if ((base * W) % 64 == 0 && base * W <= LB) {
-      a->W = W;
-      break;
-    }
-  if (!a->W) {
-    set_err(SOI_ERR_ARGS, "S=%ld has no supported filter block shape (S*n_mu*2*W must be a "
-                          "multiple of 64 and at most %d)", pl->S, LB);
+  if (soik_filter_shape(a->S, pl->n_mu, pl->d_mu, pl->B, a->ct, &a->NP, &a->BP, &a->NT, &a->U,
+                        &a->W)) {
+    set_err(SOI_ERR_ARGS, "no filter kernel for S=%ld, n_mu/d_mu=%d/%d, B=%d (S must be 4, 8, "
+            "16, 32 or 64)", pl->S, pl->n_mu, pl->d_mu, pl->B);
     return -1;
   }
-  int TJ = a->W * a->U;
-  int rows = TJ * pl->d_mu + a->NP * a->BP - pl->d_mu;
-  int grow = TJ * pl->n_mu;
-  long lds = (long)rows * a->S;
-  long g = (long)grow * (a->S + 1);
-  if (g > lds) lds = g;
-  lds *= 16;
-  if (lds > 160 * 1024) {
-    set_err(SOI_ERR_ARGS, "filter tile needs %ld bytes of LDS (> 160 KiB) at S=%ld", lds, pl->S);
-    return -1;
-  }
-  a->lds_bytes = (int)lds;
   a->M = pl->M;
   a->Mh = pl->Mh;
   a->nJ = pl->M / pl->d_mu;
-  long ntiles = (a->nJ + TJ - 1) / TJ;
-  int per_cu = (int)((160 * 1024) / lds);
-  if (per_cu < 1) per_cu = 1;
-  if (per_cu > 4) per_cu = 4;
-  long grid = 256L * per_cu;
-  if (grid > ntiles) grid = ntiles;
-  a->grid = (int)grid;
+  size_t lds = soik_filter_lds_bytes(a);
+  if (lds > 160 * 1024) {
+    set_err(SOI_ERR_ARGS, "filter tile needs %zu bytes of LDS (> 160 KiB) at S=%ld", lds, pl->S);
+    return -1;
+  }
   return 0;
 }
 
@@ -317,8 +300,8 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     set_err(SOI_ERR_TOO_SMALL, "input size too small: M/P=%ld < B=%zu", M / d->P, d->B);
     return;
   }
-  if (d->B < 2 || d->B <= (cfft_size_t)d->d_mu) {
-    set_err(SOI_ERR_ARGS, "B=%zu must exceed d_mu=%d", d->B, d->d_mu);
+  if (d->B < 2 || d->B <= (cfft_size_t)d->d_mu || d->B > 80) {
+    set_err(SOI_ERR_ARGS, "B=%zu must exceed d_mu=%d and be at most 80", d->B, d->d_mu);
     return;
   }
 
@@ -339,7 +322,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
   pl->B = (int)d->B;
   pl->Mh = (long)d->n_mu * M / d->d_mu;
   pl->l = pl->Mh / d->P;
-  if (plan_filter(pl) || plan_fft(pl)) goto fail_quiet;
+  if (plan_filter(pl, d->window_mode) || plan_fft(pl)) goto fail_quiet;
   HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
 
   /* segment boundaries p*k (pfft.c:144-147) */
@@ -349,7 +332,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
   {
     int B_pad = pl->fa.NP * pl->fa.BP;
     size_t ntap = (size_t)d->n_mu * S * B_pad;
-    HIPCHK(hipMalloc((void **)&pl->taps, ntap * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&pl->taps, ntap * (pl->fa.ct ? 2 : 1) * sizeof(double)));
     HIPCHK(hipMalloc((void **)&pl->post, (size_t)d->n_mu * S * sizeof(soi_c64_t)));
     HIPCHK(hipMalloc((void **)&d->w, (size_t)d->n_mu * d->B * S * sizeof(soi_c64_t)));
     HIPCHK(hipMalloc((void **)&d->W_inv, (size_t)M * sizeof(soi_c64_t)));
@@ -366,7 +349,8 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       pl->owns_at = 1;
     }
     KCHK(soik_make_tables(pl->taps, pl->post, (soi_c64_t *)d->w, (soi_c64_t *)d->W_inv, (int)S,
-                          d->n_mu, d->d_mu, (int)d->B, B_pad, d->tau, d->sigma, M, pl->stream));
+                          d->n_mu, d->d_mu, (int)d->B, B_pad, d->tau, d->sigma, pl->N, M, pl->Mh,
+                          pl->ref_windows, pl->fa.ct, pl->stream));
     KCHK(soik_make_twiddles(pl->twS, S, S, 1, pl->stream));
     pl->fa.taps = pl->taps;
     pl->fa.post = pl->post;

@@ -22,18 +22,19 @@ typedef struct { float x, y; } soi_c32_t;
 typedef struct SoiFilterArgs {
   const void *x;      /* input [M][S], complex64/128 */
   void *at;           /* output alpha_tilde [S][M_hat] */
-  const double *taps; /* [n_mu][S][NP*BP] */
+  const double *taps; /* [n_mu][S][NP*BP], real (ct = 0) or complex (ct = 1) */
   const soi_c64_t *post;
   const void *twS;    /* S-point twiddles */
   long M, Mh, nJ;
-  int S, n_mu, d_mu, BP, NP, U, W;
-  int grid;
-  int lds_bytes;
+  int S, n_mu, d_mu, BP, NP, NT, U, W, ct;
 } SoiFilterArgs;
 
-int soik_make_tables(double *taps, soi_c64_t *post, soi_c64_t *w_full, soi_c64_t *winv, int S,
-                     int n_mu, int d_mu, int B, int B_pad, double tau, double sigma, long M,
-                     hipStream_t st);
+int soik_make_tables(void *taps, soi_c64_t *post, soi_c64_t *w_full, soi_c64_t *winv, int S,
+                     int n_mu, int d_mu, int B, int B_pad, double tau, double sigma, long N,
+                     long M, long Mh, int ref, int ct, hipStream_t st);
+int soik_filter_shape(int S, int n_mu, int d_mu, int B, int ct, int *NP, int *BP, int *NT,
+                      int *U, int *W);
+size_t soik_filter_lds_bytes(const SoiFilterArgs *a);
 int soik_make_twiddles(soi_c64_t *tw, long n, long L, long mult, hipStream_t st);
 int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st);
 int soik_fft_single(const soi_c64_t *at, soi_c64_t *X, const soi_c64_t *tw,

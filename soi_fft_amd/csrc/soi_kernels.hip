@@ -23,14 +23,35 @@ using namespace soi;
 // Tables (init only; FP64 on the GPU)
 // =============================================================================================
 
-// Real filter taps and complex post factors.  The reference window (pfft.c:42-66) is
+// Filter taps and phase factors.  The reference window (pfft.c:42-66) is
 //   w[theta][b][i] = e^{i pi t} y(t) / mu,  t = theta*d/n - b - i/S + kappa/2,
 //   y(t) = sin(pi tau t)/(pi tau t) * exp(-pi^2 t^2 / sigma), y(0) = 1.
-// Since e^{i pi t} = e^{i pi (t+b)} (-1)^b, w = post[theta][i] * taps[theta][i][b] with
-//   taps = (-1)^b y(t)/mu  (real)  and  post = e^{i pi (theta d/n + kappa/2 - i/S)}.
-// t and the phase are formed from exact integer numerators over 2*n*S.
-__global__ void k_make_taps(double *taps, double2 *post, double2 *w_full, int S, int n_mu,
-                            int d_mu, int B, int B_pad, double tau, double sigma)
+// ref = 1: t as the reference forms it in double, ((double)theta/M_hat - (double)j/N +
+//          (double)kappa/(2M))*M with j = b*S + i (no contraction), so the table reproduces the
+//          reference's window values; ref = 0: t from its exact rational numerator over 2nS.
+// ct = 0 (real taps, exact t only): e^{i pi t} = e^{i pi (t+b)} (-1)^b, so
+//          w = post[theta][i] * taps[theta][i][b], taps = (-1)^b y(t)/mu real,
+//          post = e^{i pi (theta d/n + kappa/2 - i/S)}.
+// ct = 1: taps[theta][i][b] = w (complex), post = 1.
+// w_full receives w in the reference's logical layout [theta][b][i] (descriptor field w).
+__device__ __forceinline__ double window_y(double t, double tau, double sigma)
+{
+  if (t == 0.0) return 1.0;
+  const double PI_D = 3.14159265358979323846;
+  // sin(pi tau t) with the product tau*t carried to double-double
+  double ph = __dmul_rn(tau, t);
+  double pl = fma(tau, t, -ph);
+  double sp, cp;
+  sincospi(ph, &sp, &cp);
+  double num = fma(M_PI * pl, cp, sp);
+  double den = __dmul_rn(__dmul_rn(PI_D, tau), t);
+  double e = exp(__ddiv_rn(__dmul_rn(__dmul_rn(__dmul_rn(-PI_D, PI_D), t), t), sigma));
+  return __dmul_rn(__ddiv_rn(num, den), e);
+}
+
+__global__ void k_make_taps(void *taps, double2 *post, double2 *w_full, int S, int n_mu,
+                            int d_mu, int B, int B_pad, double tau, double sigma, long N, long M,
+                            long Mh, int ref, int ct)
 {
   long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long total = (long)n_mu * S * B_pad;
@@ -41,29 +62,43 @@ __global__ void k_make_taps(double *taps, double2 *post, double2 *w_full, int S,
   int theta = (int)(rest / S);
   const long den = 2L * n_mu * S;
   const int kappa = B - d_mu;
-  long num0 = 2L * theta * d_mu * S + (long)kappa * n_mu * S - 2L * i * n_mu;  // t + b, scaled
-  double tap = 0.0;
+  const double mu = (double)n_mu / (double)d_mu;
+  long num0 = 2L * theta * d_mu * S + (long)kappa * n_mu * S - 2L * i * n_mu;  // (t + b) * den
+  // post phase e^{i pi num0/den}: num0 reduced modulo 2*den (period 2 in units of pi)
+  long r0 = num0 % (2 * den);
+  if (r0 < 0) r0 += 2 * den;
+  double sp0, cp0;
+  sincospi((double)r0 / (double)den, &sp0, &cp0);
+  if (b == 0) post[theta * S + i] = ct ? double2{1.0, 0.0} : double2{cp0, sp0};
+  double2 wv = double2{0.0, 0.0};
   if (b < B) {
-    long num = num0 - 2L * b * n_mu * S;
-    double t = (double)num / (double)den;
-    double y;
-    if (num == 0) {
-      y = 1.0;
+    double t;
+    double sp, cp;
+    if (ref) {
+      long j = (long)b * S + i;
+      double a0 = __ddiv_rn((double)theta, (double)Mh);
+      double a1 = __ddiv_rn((double)j, (double)N);
+      double a2 = __ddiv_rn((double)kappa, (double)(2 * M));
+      t = __dmul_rn(__dadd_rn(__dsub_rn(a0, a1), a2), (double)M);
+      sincospi(t, &sp, &cp);
     } else {
-      double x = tau * t;
-      y = sinpi(x) / (M_PI * x) * exp(-M_PI * M_PI * t * t / sigma);
+      long num = num0 - 2L * b * n_mu * S;
+      t = (double)num / (double)den;
+      long rr = num % (2 * den);
+      if (rr < 0) rr += 2 * den;
+      sincospi((double)rr / (double)den, &sp, &cp);
     }
-    tap = ((b & 1) ? -y : y) * ((double)d_mu / (double)n_mu);
+    double y = window_y(t, tau, sigma);
+    wv = double2{__ddiv_rn(__dmul_rn(cp, y), mu), __ddiv_rn(__dmul_rn(sp, y), mu)};
+    if (w_full) w_full[((long)theta * B + b) * S + i] = wv;
+    if (!ct) {
+      double tap = __ddiv_rn(y, mu);
+      ((double *)taps)[idx] = (b & 1) ? -tap : tap;
+    }
+  } else if (!ct) {
+    ((double *)taps)[idx] = 0.0;
   }
-  taps[idx] = tap;
-  // phase e^{i pi num0/den}: reduce num0 modulo 2*den (period 2 in units of pi)
-  long r = num0 % (2 * den);
-  if (r < 0) r += 2 * den;
-  double sp, cp;
-  sincospi((double)r / (double)den, &sp, &cp);
-  if (b == 0) post[theta * S + i] = double2{cp, sp};
-  // full complex window in the reference's logical layout [theta][b][i] (descriptor field w)
-  if (w_full && b < B) w_full[((long)theta * B + b) * S + i] = double2{cp * tap, sp * tap};
+  if (ct) ((double2 *)taps)[idx] = wv;
 }
 
 // W_inv[m] = e^{-2 pi i kappa m / (2M)} / w_hat(m/M - 1/2)   (pfft.c:68-82)
@@ -96,117 +131,6 @@ __global__ void k_f64_to_f32(const double2 *in, float2 *out, long n)
 {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m < n) out[m] = float2{(float)in[m].x, (float)in[m].y};
-}
-
-// =============================================================================================
-// Step 1: filter + subsample + S-point DFT (FP64)
-// =============================================================================================
-// Work tile: TJ = W*U consecutive row groups J (output rows R = J*n_mu + theta) for all S lanes.
-// Thread (i, theta, p, w) owns lane i, phase theta, tap half p (taps b in [p*BP, (p+1)*BP)) and
-// J-stream w (U consecutive J).  Its BP taps stay in registers for the whole kernel; the input
-// column comes from an LDS tile of TJ*d + B_pad - d rows x S lanes, each LDS value feeding up
-// to U accumulators.  The two tap halves are summed through LDS, multiplied by post[theta][i]
-// and laid out as G[R][i]; the S-point DFT of every row is done in LDS and the tile is stored
-// segment-major, TJ*n_mu contiguous values per segment.
-// Fully unrolled polyphase accumulation: input row o of the thread's column feeds accumulator u
-// with tap b = o - u*d whenever 0 <= b < BP.  Expanded at compile time so that every tap and
-// accumulator index is a constant (taps and accumulators stay in registers).
-template <int O, int UU, int U, int DMU, int BP>
-__device__ __forceinline__ void conv_taps(const double2 xv, const double (&h)[BP], double2 (&acc)[U])
-{
-  if constexpr (UU < U) {
-    constexpr int b = O - UU * DMU;
-    if constexpr (b >= 0 && b < BP) {
-      acc[UU].x = fma(h[b], xv.x, acc[UU].x);
-      acc[UU].y = fma(h[b], xv.y, acc[UU].y);
-    }
-    conv_taps<O, UU + 1, U, DMU, BP>(xv, h, acc);
-  }
-}
-
-template <int O, int OEND, int U, int DMU, int BP>
-__device__ __forceinline__ void conv_rows(const double2 *__restrict__ col, const int S,
-                                          const double (&h)[BP], double2 (&acc)[U])
-{
-  if constexpr (O < OEND) {
-    const double2 xv = col[O * S];
-    conv_taps<O, 0, U, DMU, BP>(xv, h, acc);
-    conv_rows<O + 1, OEND, U, DMU, BP>(col, S, h, acc);
-  }
-}
-
-template <int NMU, int DMU, int BP, int U, int NP, int LB, int MINW>
-__global__ __launch_bounds__(LB, MINW) void k_filter_sdft_f64(
-    const double2 *__restrict__ x, double2 *__restrict__ at, const double *__restrict__ taps,
-    const double2 *__restrict__ post, const double2 *__restrict__ twS, long M,
-    int S, long Mh, int W, long nJ)
-{
-  extern __shared__ double2 lds[];
-  const int tid = threadIdx.x;
-  const int nt = blockDim.x;
-  const int i = tid % S;
-  int rest = tid / S;
-  const int theta = rest % NMU;
-  rest /= NMU;
-  const int p = rest % NP;
-  const int w = rest / NP;
-  const int TJ = W * U;
-  const int rows = TJ * DMU + NP * BP - DMU;  // input rows of one tile
-  const int grow = TJ * NMU;                  // output rows of one tile
-  const int gs = S + 1;                       // padded row stride of the G tile
-
-  double h[BP];
-  {
-    const double *tp = taps + ((long)theta * S + i) * (NP * BP) + p * BP;
-#pragma unroll
-    for (int b = 0; b < BP; ++b) h[b] = tp[b];
-  }
-  const double2 pf = post[theta * S + i];
-  const long ntiles = (nJ + TJ - 1) / TJ;
-
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long J0 = tile * TJ;
-    const long q0 = J0 * DMU;
-    // stage the input rows (circular: the last rank's halo is rank 0's first rows)
-    for (int e = tid; e < rows * S; e += nt) {
-      int r = e / S, ln = e - r * S;
-      long q = q0 + r;
-      if (q >= M) q -= M;
-      lds[e] = x[q * S + ln];
-    }
-    __syncthreads();
-
-    double2 acc[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) acc[u] = double2{0.0, 0.0};
-    const double2 *col = lds + (w * U * DMU + p * BP) * S + i;
-    conv_rows<0, (U - 1) * DMU + BP, U, DMU, BP>(col, S, h, acc);
-    __syncthreads();  // the input tile is dead from here on; G reuses its space
-    double2 *G = lds;
-    if (NP == 2 && p == 1) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) G[((w * U + u) * NMU + theta) * gs + i] = acc[u];
-    }
-    if (NP == 2) __syncthreads();
-    if (p == 0) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        double2 a = acc[u];
-        if (NP == 2) a = cadd(a, G[((w * U + u) * NMU + theta) * gs + i]);
-        G[((w * U + u) * NMU + theta) * gs + i] = cmul(a, pf);
-      }
-    }
-    __syncthreads();
-    // S-point DFT of each row (in place; sequence stride gs, element stride 1)
-    lds_fft<10>(G, S, grow, gs, 1, twS);
-    // segment-major store: at[s][J0*n_mu + r]
-    const long R0 = J0 * NMU;
-    for (int e = tid; e < grow * S; e += nt) {
-      int s = e / grow, r = e - s * grow;
-      if (R0 + r < Mh) at[(long)s * Mh + R0 + r] = G[r * gs + s];
-    }
-    __syncthreads();
-  }
 }
 
 // =============================================================================================
@@ -553,13 +477,14 @@ __global__ void k_gen_tones(double2 *x, long N, const double *__restrict__ tf,
 // =============================================================================================
 static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-extern "C" int soik_make_tables(double *taps, double2 *post, double2 *w_full, double2 *winv,
+extern "C" int soik_make_tables(void *taps, double2 *post, double2 *w_full, double2 *winv,
                                 int S, int n_mu, int d_mu, int B, int B_pad, double tau,
-                                double sigma, long M, hipStream_t st)
+                                double sigma, long N, long M, long Mh, int ref, int ct,
+                                hipStream_t st)
 {
   long nt = (long)n_mu * S * B_pad;
   hipLaunchKernelGGL(k_make_taps, dim3(nblk(nt, 256)), dim3(256), 0, st, taps, post, w_full, S,
-                     n_mu, d_mu, B, B_pad, tau, sigma);
+                     n_mu, d_mu, B, B_pad, tau, sigma, N, M, Mh, ref, ct);
   hipLaunchKernelGGL(k_make_winv, dim3(nblk(M, 256)), dim3(256), 0, st, winv, M, B - d_mu, tau,
                      sigma);
   return (int)hipGetLastError();
@@ -569,50 +494,6 @@ extern "C" int soik_make_twiddles(double2 *tw, long n, long L, long mult, hipStr
 {
   hipLaunchKernelGGL(k_make_twiddles, dim3(nblk(n, 256)), dim3(256), 0, st, tw, n, L, mult);
   return (int)hipGetLastError();
-}
-
-// Filter kernel configurations: (n_mu, d_mu) in {(5,4), (8,7)}, BP in {16,24,32,36,40}.
-// (5,4) blocks have up to 640 threads and are register-capped for 3 waves per SIMD;
-// (8,7) blocks have up to 1024 threads (S = 64 at n_mu = 8).
-template <int NMU, int DMU, int BP, int U>
-static int launch_filter(const SoiFilterArgs *a, hipStream_t st)
-{
-  constexpr int LB = (NMU == 5) ? 640 : 1024;
-  constexpr int MINW = (NMU == 5) ? 3 : 4;
-  const int threads = a->S * NMU * a->NP * a->W;
-  if (threads > LB || threads % 64) return (int)hipErrorInvalidValue;
-  auto k2 = k_filter_sdft_f64<NMU, DMU, BP, U, 2, LB, MINW>;
-  auto k1 = k_filter_sdft_f64<NMU, DMU, BP, U, 1, LB, MINW>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    (void)hipFuncSetAttribute((const void *)k1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr_set = true;
-  }
-  auto kern = (a->NP == 2) ? k2 : k1;
-  hipLaunchKernelGGL(kern, dim3(a->grid), dim3(threads), a->lds_bytes, st, (const double2 *)a->x,
-                     (double2 *)a->at, a->taps, a->post, (const double2 *)a->twS, a->M, a->S,
-                     a->Mh, a->W, a->nJ);
-  return (int)hipGetLastError();
-}
-
-extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
-{
-#define SOI_BP_CASES(NMU, DMU, U)                                  \
-  switch (a->BP) {                                                 \
-    case 16: return launch_filter<NMU, DMU, 16, U>(a, st);         \
-    case 24: return launch_filter<NMU, DMU, 24, U>(a, st);         \
-    case 32: return launch_filter<NMU, DMU, 32, U>(a, st);         \
-    case 36: return launch_filter<NMU, DMU, 36, U>(a, st);         \
-    case 40: return launch_filter<NMU, DMU, 40, U>(a, st);         \
-    default: return (int)hipErrorInvalidValue;                     \
-  }
-  if (a->n_mu == 5 && a->d_mu == 4 && a->U == 16) { SOI_BP_CASES(5, 4, 16) }
-  if (a->n_mu == 8 && a->d_mu == 7 && a->U == 8) { SOI_BP_CASES(8, 7, 8) }
-#undef SOI_BP_CASES
-  return (int)hipErrorInvalidValue;
 }
 
 static void set_lds_attr(const void *f)

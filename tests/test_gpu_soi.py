@@ -1,0 +1,245 @@
+"""Parity of the HIP path (through the soi.h C ABI) with the oracle, rocFFT and closed forms.
+
+Tolerance: relative L2 <= 1e-12 against the oracle (the FP64 bar of the plain form).  At N = 2^27
+the oracle is too slow, so the full-size checks use rocFFT's FP64 transform
+(torch.fft.fft) and the exact DFT of the tone input at sampled bins."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from closed_forms import tones_dft
+from gpu_util import d2h, rel, to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def parity_tol(orc, p):
+    """1e-12, or the agreement two FP64 implementations can reach when the demodulation gain
+    G = max|W_inv| (9.06 for the default window) amplifies rounding: 30 * eps * G."""
+    gain = float(np.abs(orc.w_inv_table(p)).max())
+    return max(TOL, 30 * 2.0 ** -52 * gain)
+
+
+def run_soi(soi, torch, x, P, k=1, **kw):
+    d = soi.make_descriptor(x.size, P, k, **kw)
+    try:
+        xt = to_dev(torch, x)
+        torch.cuda.synchronize()
+        soi.compute_soi(d, xt.data_ptr())
+        return xt.cpu().numpy(), list(d.segmentBoundaries[: P + 1])
+    finally:
+        soi.free_soi_descriptor(d)
+
+
+SHAPES = [
+    # N, P, k, extra descriptor fields
+    (2 ** 13, 4, 1, {}),
+    (2 ** 16, 8, 1, {}),
+    (2 ** 16, 4, 2, {}),
+    (2 ** 20, 8, 1, {}),                       # BASELINE config 1 shape
+    (3 * 2 ** 14, 4, 1, {}),
+    (5 * 2 ** 14, 8, 1, {}),
+    (2 ** 18, 16, 1, {}),
+    (2 ** 18, 32, 1, {}),
+    (2 ** 17, 8, 1, dict(B=76, tau=872 / 1024, sigma=313.1715)),
+    (2 ** 17, 8, 1, dict(B=64, tau=818 / 1024, sigma=267.4513)),
+    (2 ** 17, 8, 1, dict(B=44, tau=512 / 1024, sigma=121.0936)),
+    (2 ** 16, 8, 1, dict(B=30, tau=0.154, sigma=73.2102363)),
+    (7 * 2 ** 13, 4, 1, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),
+    (7 * 2 ** 15, 8, 1, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),
+]
+
+
+@pytest.mark.parametrize("N,P,k,kw", SHAPES)
+def test_soi_matches_oracle(soi, orc, torch_cuda, N, P, k, kw):
+    print(f"case N={N} P={P} k={k} {kw}", flush=True)
+    x = orc.uniform_input(N, 101)
+    out, sb = run_soi(soi, torch_cuda, x, P, k, **kw)
+    okw = {kk: v for kk, v in kw.items() if kk in ("n_mu", "d_mu", "B", "tau", "sigma")}
+    p = orc.params(N, P, k, **okw)
+    ref, osb = orc.compute_soi(p, x)
+    assert rel(out, ref) < parity_tol(orc, p)
+    assert sb == list(osb)
+    # the SOI output is the DFT: check against numpy within the window's truncation error
+    exact = np.fft.fft(x)
+    bound = 3e-14 if kw.get("n_mu", 5) == 5 and kw.get("B", 72) >= 64 else 1e-7
+    assert rel(out, exact) < bound
+
+
+def test_filter_stage_matches_oracle(soi, orc, torch_cuda):
+    torch = torch_cuda
+    N, P = 2 ** 16, 8
+    x = orc.uniform_input(N, 5)
+    d = soi.SoiDesc()
+    soi.set_default_soi_descriptor(d)
+    d.N, d.P = N, P
+    at = torch.empty(d.S * d.M_hat, dtype=torch.complex128, device="cuda")
+    d.alpha_tilde = at.data_ptr()          # caller-preallocated buffer (pfft.c:154-176)
+    soi.init_soi_descriptor(d, 1)
+    try:
+        xt = to_dev(torch, x)
+        soi.parallel_filter_subsampling(d, xt.data_ptr())
+        got = at.cpu().numpy().reshape(d.S, d.M_hat)
+        ref = orc.filter_subsample(orc.params(N, P), x)
+        assert rel(got, ref) < 1e-13
+        # the input is read-only for Step 1
+        assert np.array_equal(xt.cpu().numpy(), x)
+    finally:
+        soi.free_soi_descriptor(d)
+
+
+def test_window_tables_match_oracle(soi, orc, torch_cuda):
+    N, P = 2 ** 16, 8
+    d = soi.make_descriptor(N, P)
+    try:
+        S, M = d.S, d.M
+        w = d2h(d.w, 2 * d.n_mu * d.B * S, np.float64).view(np.complex128)
+        winv = d2h(d.W_inv, 2 * M, np.float64).view(np.complex128)
+    finally:
+        soi.free_soi_descriptor(d)
+    p = orc.params(N, P)
+    orc.set_window_mode(1)
+    try:
+        wref = orc.w_table(p).reshape(-1)
+    finally:
+        orc.set_window_mode(0)
+    assert np.abs(w - wref).max() / np.abs(wref).max() < 1e-15
+    assert rel(winv, orc.w_inv_table(p)) < 1e-15
+
+
+def test_host_pointer_path(soi, orc, torch_cuda):
+    N, P = 2 ** 16, 8
+    x = orc.uniform_input(N, 9)
+    buf = np.ascontiguousarray(x.copy())
+    d = soi.make_descriptor(N, P)
+    try:
+        soi.compute_soi(d, buf.ctypes.data)
+    finally:
+        soi.free_soi_descriptor(d)
+    ref, _ = orc.compute_soi(orc.params(N, P), x)
+    assert rel(buf, ref) < TOL
+
+
+def test_deterministic_and_reusable(soi, orc, torch_cuda):
+    torch = torch_cuda
+    N, P = 2 ** 18, 16
+    x = orc.uniform_input(N, 3)
+    d = soi.make_descriptor(N, P)
+    try:
+        outs = []
+        for _ in range(3):
+            xt = to_dev(torch, x)
+            soi.compute_soi(d, xt.data_ptr())
+            outs.append(xt.cpu().numpy())
+    finally:
+        soi.free_soi_descriptor(d)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_async_on_caller_stream(soi, orc, torch_cuda):
+    torch = torch_cuda
+    N, P = 2 ** 16, 8
+    x = orc.uniform_input(N, 4)
+    d = soi.make_descriptor(N, P)
+    try:
+        s = torch.cuda.Stream()
+        xt = to_dev(torch, x)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            soi.compute_soi_async(d, xt.data_ptr(), s.cuda_stream)
+        s.synchronize()
+        got = xt.cpu().numpy()
+    finally:
+        soi.free_soi_descriptor(d)
+    ref, _ = orc.compute_soi(orc.params(N, P), x)
+    assert rel(got, ref) < TOL
+
+
+def test_device_generator_matches_oracle(soi, orc, torch_cuda):
+    torch = torch_cuda
+    N = 12345
+    xt = torch.empty(N, dtype=torch.complex128, device="cuda")
+    assert soi.lib().soi_fill_uniform(xt.data_ptr(), N, 77, None) == 0
+    assert np.array_equal(xt.cpu().numpy(), orc.uniform_input(N, 77))
+
+
+def _config2_tones(seed, n=128, N=2 ** 27):
+    rng = np.random.default_rng(seed)
+    f = np.floor(rng.uniform(0, N, n)) + rng.uniform(0, 1, n)
+    ph = rng.uniform(0, 1, n)
+    return f, ph
+
+
+def test_config2_full_size_vs_rocfft(soi, torch_cuda):
+    """N = 2^27, P = 32, 128 unit tones + Gaussian noise (sigma 1e-3)."""
+    torch = torch_cuda
+    N, P = 2 ** 27, 32
+    f, ph = _config2_tones(2027)
+    xt = torch.empty(N, dtype=torch.complex128, device="cuda")
+    rc = soi.lib().soi_fill_tones(xt.data_ptr(), N, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  ph.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(f), 1.0,
+                                  1e-3, 2027, None)
+    assert rc == 0
+    ref = torch.fft.fft(xt)
+    e_in = torch.sum(torch.abs(xt) ** 2).item()
+    d = soi.make_descriptor(N, P)
+    try:
+        torch.cuda.synchronize()
+        soi.compute_soi(d, xt.data_ptr())
+    finally:
+        soi.free_soi_descriptor(d)
+    err = (torch.linalg.norm(xt - ref) / torch.linalg.norm(ref)).item()
+    print("config2 rel-L2 vs rocFFT", err, flush=True)
+    assert err < TOL
+    # Parseval: sum |X|^2 = N sum |x|^2
+    assert abs(torch.sum(torch.abs(xt) ** 2).item() / (N * e_in) - 1) < 1e-12
+
+
+def test_config2_tones_closed_form(soi, torch_cuda):
+    """Noise-free tones at N = 2^27: exact DFT at every tone bin and 4096 random bins."""
+    torch = torch_cuda
+    N, P = 2 ** 27, 32
+    f, ph = _config2_tones(7)
+    amps = np.ones(len(f))
+    xt = torch.empty(N, dtype=torch.complex128, device="cuda")
+    rc = soi.lib().soi_fill_tones(xt.data_ptr(), N, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  ph.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(f), 1.0,
+                                  0.0, 0, None)
+    assert rc == 0
+    d = soi.make_descriptor(N, P)
+    try:
+        torch.cuda.synchronize()
+        soi.compute_soi(d, xt.data_ptr())
+    finally:
+        soi.free_soi_descriptor(d)
+    rng = np.random.default_rng(1)
+    bins = np.unique(np.concatenate([np.floor(f).astype(np.int64), np.ceil(f).astype(np.int64) % N,
+                                     rng.integers(0, N, 4096)]))
+    got = xt[torch.from_numpy(bins).cuda()].cpu().numpy()
+    exact = tones_dft(N, f, ph, amps, bins)
+    assert rel(got, exact) < 1e-12
+
+
+@pytest.mark.parametrize("N,P,kw", [
+    (2 ** 18, 32, dict()),                                          # B=72: reference windows
+    (2 ** 17, 8, dict(B=44, tau=512 / 1024, sigma=121.0936)),       # exact windows, narrow
+    (7 * 2 ** 13, 4, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),
+])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_window_modes(soi, orc, torch_cuda, N, P, kw, mode):
+    """SOI_WINDOW_REFERENCE reproduces the reference's window values (oracle default);
+    SOI_WINDOW_EXACT uses exact t (oracle window mode 1)."""
+    x = orc.uniform_input(N, 33)
+    out, _ = run_soi(soi, torch_cuda, x, P, window_mode=mode, **kw)
+    okw = {kk: v for kk, v in kw.items() if kk in ("n_mu", "d_mu", "B", "tau", "sigma")}
+    p = orc.params(N, P, **okw)
+    try:
+        orc.set_window_mode(0 if mode == 1 else 1)
+        ref, _ = orc.compute_soi(p, x)
+    finally:
+        orc.set_window_mode(0)
+    assert rel(out, ref) < max(2e-14, parity_tol(orc, p) / 10)

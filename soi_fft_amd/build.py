@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libsoi_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("SOI_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["soi_kernels.hip", "soi_filter.hip"]
+HIP_SOURCES = ["soi_kernels.hip", "soi_filter.hip", "soi_fft.hip"]
 C_SOURCES = ["pfft.c", "compress_host.c"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
@@ -44,13 +44,19 @@ def build(verbose=False):
     cc = shutil.which("gcc") or "cc"
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in HIP_SOURCES:
+    procs = []
+    for src in HIP_SOURCES:  # the HIP translation units compile in parallel
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src + ".o")
-        _run([hipcc] + HIPCC_FLAGS + ["-c", path, "-o", obj])
+        cmd = [hipcc] + HIPCC_FLAGS + ["-c", path, "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
         objs.append(obj)
+    for cmd, p in procs:
+        out = p.communicate()[0]
+        if p.returncode != 0:
+            sys.stderr.write("\n".join(l for l in out.splitlines() if "packed-fp32" not in l) + "\n")
+            raise RuntimeError("build step failed: " + " ".join(cmd))
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         _run([cc] + CC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])

@@ -81,8 +81,8 @@ typedef struct {
   void *taps;
   soi_c64_t *post, *twS;
   /* fft */
-  int fft_single, single_count;
-  int n1, n2, Tc, Tr, B1;
+  int fft_single, single_count, fft_reg;
+  int n1, n2, Tc, Tr, B1, lb1;
   soi_c64_t *tw1, *tw2, *thi, *tlo;
   /* buffers */
   soi_c64_t *io;        /* staging for host-pointer inputs */
@@ -195,8 +195,40 @@ static int plan_fft(soi_plan_t *pl)
     return 0;
   }
   pl->fft_single = 0;
-  /* n1 x n2 = Mh, both smooth, column tile Tc >= 2 and row tile Tr >= 2 preferred; minimise
-   * max(n1, n2) with n1 a power of two when possible */
+  /* register kernels: columns of n1 and rows of n2 both have one, 8-wide tiles on both sides */
+  {
+    long best = LONG_MAX;
+    int bn1 = 0;
+    for (long n1 = 8; n1 <= 4096; n1 += 8) {
+      if (Mh % n1) continue;
+      long n2 = Mh / n1;
+      if (n2 % 8 || n2 > 4096) continue;
+      if (!soik_fft_reg_supported((int)n1, 0) || !soik_fft_reg_supported((int)n2, 1)) continue;
+      long cost = (n1 > n2 ? n1 : n2);
+      if (cost < best) {
+        best = cost;
+        bn1 = (int)n1;
+      }
+    }
+    if (bn1) {
+      pl->fft_reg = 1;
+      pl->n1 = bn1;
+      pl->n2 = (int)(Mh / bn1);
+      /* tile widths: 8 by default; SOI_FFT_TC / SOI_FFT_TR (4 or 8) override for tuning */
+      pl->Tc = pl->Tr = 8;
+      const char *ec = getenv("SOI_FFT_TC"), *er = getenv("SOI_FFT_TR");
+      if (ec && (atoi(ec) == 4 || atoi(ec) == 8)) pl->Tc = atoi(ec);
+      if (er && (atoi(er) == 4 || atoi(er) == 8)) pl->Tr = atoi(er);
+      if (pl->n2 % pl->Tc || pl->n1 % pl->Tr) pl->Tc = pl->Tr = 8;
+      int lb = 0;
+      while ((1L << lb) * (1L << lb) < Mh) ++lb;
+      pl->B1 = 1 << lb;
+      pl->lb1 = lb;
+      return 0;
+    }
+  }
+  /* generic LDS kernels: n1 x n2 = Mh, both smooth, column tile Tc >= 2 and row tile Tr >= 2
+   * preferred; minimise max(n1, n2) with n1 a power of two when possible */
   long best_cost = LONG_MAX;
   int bn1 = 0, bn2 = 0, bTc = 0, bTr = 0;
   for (long n1 = 2; n1 <= CAP; ++n1) {
@@ -224,9 +256,10 @@ static int plan_fft(soi_plan_t *pl)
   pl->n2 = bn2;
   pl->Tc = bTc;
   pl->Tr = bTr;
-  int b1 = 1;
-  while ((long)b1 * b1 < Mh) b1 <<= 1;
+  int b1 = 1, lb = 0;
+  while ((long)b1 * b1 < Mh) b1 <<= 1, ++lb;
   pl->B1 = b1;
+  pl->lb1 = lb;
   return 0;
 }
 
@@ -535,6 +568,12 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, soi_c64_t *X, hipStream_t st)
     KCHK(soik_fft_single(at, X, pl->tw1, (const soi_c64_t *)d->W_inv, pl->Mh, pl->M, (int)pl->S,
                          pl->single_count, st));
     mark(pl, 3);
+  } else if (pl->fft_reg) {
+    KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
+                           (int)pl->S, pl->Tc, st));
+    mark(pl, 3);
+    KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Mh,
+                           pl->M, (int)pl->S, pl->Tr, st));
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
                       (int)pl->S, st));

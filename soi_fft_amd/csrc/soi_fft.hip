@@ -1,0 +1,401 @@
+// soi_fft.hip -- M_hat-point segment FFTs of the SOI FFT on MI355X (reference: the DFTI transform
+// pfft.c:227-228 runs at pfft.c:642, then the demodulation pfft.c:649-693).
+//
+// Four-step split M_hat = n1 * n2, R = n2*r1 + r2, m = m1 + n1*m2:
+//   pass 1 (columns): for each column r2, Z[r2][m1] = sum_r1 x[n2 r1 + r2] w_n1^{r1 m1}, times the
+//                     twiddle w_Mhat^{r2 m1}, written back in place at n2*m1 + r2;
+//   pass 2 (rows):    y[m1 + n1 m2] = sum_r2 Z[r2][m1] w_n2^{r2 m2}, demodulated and stored as
+//                     X[s*M + m] = W_inv[m] * y[m] for m < M.
+// Both passes keep their transforms in registers (E values per thread, Stockham stages with the
+// radices of the template list) and exchange through LDS between stages.  A block takes T = 8
+// adjacent columns (pass 1) or rows (pass 2) so every global access moves 128-byte runs; blocks
+// are remapped so neighbouring tiles share an XCD (and its L2).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "soi_common.hpp"
+#include "soi_kernels.h"
+
+using namespace soi;
+
+// ---------------------------------------------------------------------------------------------
+// radix-10 and radix-16 forward DFTs in registers
+// ---------------------------------------------------------------------------------------------
+template <typename C> __device__ __forceinline__ void dft16(C *a)
+{
+  typedef decltype(a[0].x) T;
+  // 16 = 4 x 4: columns n = n1 + 4 n2, first DFT4 over n2, twiddle w16^{n1 k2}, DFT4 over n1
+  C b[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    C x0 = a[n1], x1 = a[n1 + 4], x2 = a[n1 + 8], x3 = a[n1 + 12];
+    dft4(x0, x1, x2, x3);
+    b[n1 * 4 + 0] = x0;
+    b[n1 * 4 + 1] = x1;
+    b[n1 * 4 + 2] = x2;
+    b[n1 * 4 + 3] = x3;
+  }
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  const T r2 = (T)0.70710678118654752440;
+  // twiddles w16^{n1 k2} = e^{-2 pi i n1 k2 / 16}
+  // n1 = 1: k2 = 1,2,3 -> w^1, w^2, w^3 ; n1 = 2: w^2, w^4, w^6 ; n1 = 3: w^3, w^6, w^9
+  const C w1 = C{c1, -s1}, w2 = C{r2, -r2}, w3 = C{s1, -c1};
+  const C w6 = C{-r2, -r2}, w9 = C{-c1, s1};
+  b[1 * 4 + 1] = cmul(b[5], w1);
+  b[1 * 4 + 2] = cmul(b[6], w2);
+  b[1 * 4 + 3] = cmul(b[7], w3);
+  b[2 * 4 + 1] = cmul(b[9], w2);
+  b[2 * 4 + 2] = cmul_mi(b[10]);
+  b[2 * 4 + 3] = cmul(b[11], w6);
+  b[3 * 4 + 1] = cmul(b[13], w3);
+  b[3 * 4 + 2] = cmul(b[14], w6);
+  b[3 * 4 + 3] = cmul(b[15], w9);
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    C x0 = b[0 * 4 + k2], x1 = b[1 * 4 + k2], x2 = b[2 * 4 + k2], x3 = b[3 * 4 + k2];
+    dft4(x0, x1, x2, x3);
+    // output k = k2 + 4 k1
+    a[k2 + 0] = x0;
+    a[k2 + 4] = x1;
+    a[k2 + 8] = x2;
+    a[k2 + 12] = x3;
+  }
+}
+
+template <typename C> __device__ __forceinline__ void dft10(C *a)
+{
+  typedef decltype(a[0].x) T;
+  // 10 = 2 x 5 (Good-Thomas free form): n = n1 + 2 n2 ... use n = 5 n1 + n2? Use Cooley-Tukey:
+  // n = n1 + 2 n2 (n1 < 2, n2 < 5): DFT5 over n2 for each n1, twiddle w10^{n1 k2}, DFT2 over n1.
+  C e[5] = {a[0], a[2], a[4], a[6], a[8]};
+  C o[5] = {a[1], a[3], a[5], a[7], a[9]};
+  dft5(e[0], e[1], e[2], e[3], e[4]);
+  dft5(o[0], o[1], o[2], o[3], o[4]);
+  const T c1 = (T)0.80901699437494742410, s1 = (T)0.58778525229247312917;  // cos, sin 36 deg
+  const T c2 = (T)0.30901699437494742410, s2 = (T)0.95105651629515357212;  // cos, sin 72 deg
+  o[1] = cmul(o[1], C{c1, -s1});
+  o[2] = cmul(o[2], C{c2, -s2});
+  o[3] = cmul(o[3], C{-c2, -s2});
+  o[4] = cmul(o[4], C{-c1, -s1});
+#pragma unroll
+  for (int k2 = 0; k2 < 5; ++k2) {
+    a[k2] = cadd(e[k2], o[k2]);
+    a[k2 + 5] = csub(e[k2], o[k2]);
+  }
+}
+
+template <int R, typename C> __device__ __forceinline__ void dft_any(C *a)
+{
+  if constexpr (R == 16) dft16(a);
+  else if constexpr (R == 10) dft10(a);
+  else dft_r<R>(a);
+}
+
+// ---------------------------------------------------------------------------------------------
+// compile-time radix schedules
+// ---------------------------------------------------------------------------------------------
+template <int... Rs> struct Radices {
+  static constexpr int n = sizeof...(Rs);
+  static constexpr int r[n] = {Rs...};
+  static constexpr int at(int s) { return r[s]; }
+  static constexpr int span(int s)  // product of radices before stage s
+  {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= r[i];
+    return p;
+  }
+};
+
+// LDS index of position pos of sequence q (q fastest), one pad slot per 16 positions
+template <int T> __device__ __forceinline__ int lidx(int pos, int q) { return (pos + (pos >> 4)) * T + q; }
+
+// Exchange a thread's stage outputs through LDS into the next stage's input order.  Outputs of
+// stage s (radix R, span Ns) for butterfly j land at (j - j%Ns)*R + j%Ns + k*Ns; stage s+1
+// (radix R2) butterfly j2 reads j2 + k*L/R2.  Done in parts: real then imaginary halves, and
+// NQP groups of the T sequences, so the LDS footprint is L*(T/NQP)*8 bytes (plus padding).
+template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2>
+__device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, int ts)
+{
+  constexpr int NB = E / R, NB2 = E / R2, TG = T / NQP;
+  const int qg = q / TG, ql = q % TG;
+#pragma unroll
+  for (int g = 0; g < NQP; ++g) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (qg == g) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          const int j = ts + TS * c;
+          const int jm = j % NS;
+          const int base = (j - jm) * R + jm;
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            lds[lidx<TG>(base + k * NS, ql)] = half ? v[c * R + k].y : v[c * R + k].x;
+        }
+      }
+      __syncthreads();
+      if (qg == g) {
+#pragma unroll
+        for (int c = 0; c < NB2; ++c) {
+          const int j = ts + TS * c;
+#pragma unroll
+          for (int k = 0; k < R2; ++k) {
+            const double val = lds[lidx<TG>(j + k * (L / R2), ql)];
+            if (half) v[c * R2 + k].y = val; else v[c * R2 + k].x = val;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// twiddle + DFT of every butterfly of stage s held in registers
+template <int L, int E, int TS, int R, int NS>
+__device__ __forceinline__ void stage(double2 (&v)[E], int ts, const double2 *__restrict__ tw)
+{
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    const int j = ts + TS * c;
+    if constexpr (NS > 1) {
+      const int jm = j % NS;
+      constexpr int step = L / (NS * R);
+#pragma unroll
+      for (int k = 1; k < R; ++k) v[c * R + k] = cmul(v[c * R + k], tw[k * jm * step]);
+    }
+    dft_any<R>(&v[c * R]);
+  }
+}
+
+// Run stages 0..n-1 on register data already loaded for stage 0.
+template <int L, int E, int T, int NQP, int TS, typename RS, int S0>
+__device__ __forceinline__ void run_stages(double2 (&v)[E], double *lds, int q, int ts,
+                                           const double2 *__restrict__ tw)
+{
+  constexpr int R = RS::at(S0);
+  constexpr int NS = RS::span(S0);
+  stage<L, E, TS, R, NS>(v, ts, tw);
+  if constexpr (S0 + 1 < RS::n) {
+    exchange<L, E, T, NQP, TS, R, NS, RS::at(S0 + 1)>(v, lds, q, ts);
+    run_stages<L, E, T, NQP, TS, RS, S0 + 1>(v, lds, q, ts, tw);
+  }
+}
+
+// After the last stage, value v[c*R + k] of butterfly j = ts + TS*c is output index
+// (j - j%NS)*R + j%NS + k*NS with NS = L/R.
+template <int L, int TS, typename RS>
+__device__ __forceinline__ int out_index(int ts, int c, int k)
+{
+  constexpr int R = RS::at(RS::n - 1);
+  constexpr int NS = L / R;
+  const int j = ts + TS * c;
+  const int jm = j % NS;
+  return (j - jm) * R + jm + k * NS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pass 1: T adjacent columns of length L = n1 (stride n2), twiddled, in place
+// ---------------------------------------------------------------------------------------------
+// waves per SIMD the register budget is sized for: two blocks per CU for the 4-wide tiles, one
+// block otherwise
+constexpr int fft_minw(int T, int threads) { return ((T == 4 ? 2 : 1) * threads / 64 + 3) / 4; }
+
+template <int L, int E, int T, int NQP, typename RS>
+__global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_cols(double2 *__restrict__ data,
+                                                          const double2 *__restrict__ tw,
+                                                          const double2 *__restrict__ thi,
+                                                          const double2 *__restrict__ tlo,
+                                                          int n2, int lb1, long Mh, int S)
+{
+  constexpr int TS = L / E;
+  extern __shared__ double ldsd[];
+  const int tid = threadIdx.x;
+  const int q = tid % T, ts = tid / T;
+  const int ncb = n2 / T;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = bid % ncb, s = bid / ncb;
+  if (s >= S) return;
+  const int c0 = cb * T;
+  double2 *seg = data + (long)s * Mh + c0 + q;
+  const int n2s = n2;
+
+  double2 v[E];
+  constexpr int R0 = RS::at(0);
+#pragma unroll
+  for (int c = 0; c < E / R0; ++c)
+#pragma unroll
+    for (int k = 0; k < R0; ++k) v[c * R0 + k] = seg[(unsigned)(ts + TS * c + k * (L / R0)) * (unsigned)n2s];
+
+  run_stages<L, E, T, NQP, TS, RS, 0>(v, ldsd, q, ts, tw);
+
+  constexpr int RL = RS::at(RS::n - 1);
+#pragma unroll
+  for (int c = 0; c < E / RL; ++c)
+#pragma unroll
+    for (int k = 0; k < RL; ++k) {
+      const int m1 = out_index<L, TS, RS>(ts, c, k);
+      const unsigned ex = (unsigned)(c0 + q) * (unsigned)m1;   // < M_hat < 2^32
+      const double2 t = cmul(thi[ex >> lb1], tlo[ex & ((1u << lb1) - 1)]);
+      seg[(unsigned)m1 * (unsigned)n2s] = cmul(v[c * RL + k], t);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pass 2: T adjacent rows of length L = n2, demodulated, stored transposed with 128 B runs
+// ---------------------------------------------------------------------------------------------
+template <int L, int E, int T, int NQP, typename RS>
+__global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_rows(const double2 *__restrict__ data,
+                                                          double2 *__restrict__ X,
+                                                          const double2 *__restrict__ tw,
+                                                          const double2 *__restrict__ winv,
+                                                          int n1, long Mh, long M, int S)
+{
+  constexpr int TS = L / E;
+  extern __shared__ double ldsd[];
+  const int tid = threadIdx.x;
+  // loads: consecutive threads walk a row (coalesced); the sequence index is the slow one
+  const int ts = tid % TS, q = tid / TS;
+  const int nrb = n1 / T;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = bid % S;  // the S segments of one row block share W_inv entries
+  const int rb = bid / S;
+  if (rb >= nrb) return;
+  const int a = rb * T;
+  const double2 *row = data + (long)s * Mh + (long)(a + q) * L;
+
+  double2 v[E];
+  constexpr int R0 = RS::at(0);
+#pragma unroll
+  for (int c = 0; c < E / R0; ++c)
+#pragma unroll
+    for (int k = 0; k < R0; ++k) v[c * R0 + k] = row[ts + TS * c + k * (L / R0)];
+
+  run_stages<L, E, T, NQP, TS, RS, 0>(v, ldsd, q, ts, tw);
+
+  // transpose through LDS so that stores run over the T rows (m = a + q + n1*m2), in NQP groups
+  // of T/NQP rows and real/imaginary halves
+  constexpr int RL = RS::at(RS::n - 1);
+  constexpr int TG = T / NQP;
+  constexpr int NT = T * TS;
+  constexpr int EG = E / NQP;  // values each thread stores per group
+  double2 *out = X + (long)s * M;
+  const int qg = q / TG, ql = q % TG;
+#pragma unroll
+  for (int g = 0; g < NQP; ++g) {
+    double2 w[EG];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (qg == g) {
+#pragma unroll
+        for (int c = 0; c < E / RL; ++c)
+#pragma unroll
+          for (int k = 0; k < RL; ++k)
+            ldsd[lidx<TG>(out_index<L, TS, RS>(ts, c, k), ql)] = half ? v[c * RL + k].y : v[c * RL + k].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < EG; ++c) {
+        const int e = tid + c * NT;  // e = m2*TG + qq
+        const double val = ldsd[lidx<TG>(e / TG, e % TG)];
+        if (half) w[c].y = val; else w[c].x = val;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < EG; ++c) {
+      const int e = tid + c * NT;
+      const long m = (long)(a + g * TG + e % TG) + (long)n1 * (e / TG);
+      if (m < M) out[m] = cmul(winv[m], w[c]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------------------------
+template <int L, int T, int NQP> constexpr size_t fft_lds_bytes() { return (size_t)(L + L / 16) * (T / NQP) * 8; }
+
+template <int L, int E, int T, int NQP, typename RS>
+static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, const double2 *tlo,
+                       int n2, int lb1, long Mh, int S, hipStream_t st)
+{
+  auto kern = k_fft_cols<L, E, T, NQP, RS>;
+  constexpr size_t lds = fft_lds_bytes<L, T, NQP>();
+  static_assert(lds <= 160 * 1024, "LDS");
+  static bool once = false;
+  if (!once) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    once = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n2 / T) * S)), dim3(T * (L / E)), lds, st, data, tw,
+                     thi, tlo, n2, lb1, Mh, S);
+  return (int)hipGetLastError();
+}
+
+template <int L, int E, int T, int NQP, typename RS>
+static int launch_rows(const double2 *data, double2 *X, const double2 *tw, const double2 *winv,
+                       int n1, long Mh, long M, int S, hipStream_t st)
+{
+  auto kern = k_fft_rows<L, E, T, NQP, RS>;
+  constexpr size_t lds = fft_lds_bytes<L, T, NQP>();
+  static_assert(lds <= 160 * 1024, "LDS");
+  static bool once = false;
+  if (!once) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    once = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / T) * S)), dim3(T * (L / E)), lds, st, data, X,
+                     tw, winv, n1, Mh, M, S);
+  return (int)hipGetLastError();
+}
+
+// Register-FFT lengths (E values per thread, radices in stage order):
+//   256 = 16.16, 512 = 16.16.2, 1024 = 16.16.4, 2048 = 16.16.8            (E = 16)
+//   320 = 10.4.4.2, 640 = 10.4.4.4, 1280 = 10.4.4.4.2, 2560 = 10.4.4.4.4   (E = 20)
+// Columns take any of them (2560 exchanges in two sequence groups); rows take up to 2048.
+extern "C" int soik_fft_reg_supported(int L, int rows)
+{
+  switch (L) {
+    case 256: case 320: case 512: case 640: case 1024: case 1280: case 2048:
+      return 1;
+    case 2560:
+      return !rows;
+    default:
+      return 0;
+  }
+}
+
+// Tile width T: 8 (128-byte runs, one 1024-thread block per CU for the long lengths) or 4
+// (64-byte runs, two blocks per CU).
+extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
+                                 const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
+                                 int T, hipStream_t st)
+{
+  if (n2 % T) return (int)hipErrorInvalidValue;
+#define COLS(LV, EV, TV, NQ, ...) \
+  if (n1 == LV && T == TV) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+  COLS(256, 16, 8, 1, 16, 16) COLS(320, 20, 8, 1, 10, 4, 4, 2) COLS(512, 16, 8, 1, 16, 16, 2)
+  COLS(640, 20, 8, 1, 10, 4, 4, 4) COLS(1024, 16, 8, 1, 16, 16, 4)
+  COLS(1280, 20, 8, 1, 10, 4, 4, 4, 2) COLS(2048, 16, 8, 1, 16, 16, 8)
+  COLS(2560, 20, 8, 2, 10, 4, 4, 4, 4)
+  COLS(1024, 16, 4, 1, 16, 16, 4) COLS(1280, 20, 4, 1, 10, 4, 4, 4, 2)
+  COLS(2048, 16, 4, 1, 16, 16, 8) COLS(2560, 20, 4, 2, 10, 4, 4, 4, 4)
+#undef COLS
+  return (int)hipErrorInvalidValue;
+}
+
+extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 *tw2,
+                                 const double2 *winv, int n1, int n2, long Mh, long M, int S,
+                                 int T, hipStream_t st)
+{
+  if (n1 % T) return (int)hipErrorInvalidValue;
+#define ROWS(LV, EV, TV, NQ, ...) \
+  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, Radices<__VA_ARGS__>>(data, X, tw2, winv, n1, Mh, M, S, st);
+  ROWS(256, 16, 8, 1, 16, 16) ROWS(320, 20, 8, 1, 10, 4, 4, 2) ROWS(512, 16, 8, 1, 16, 16, 2)
+  ROWS(640, 20, 8, 1, 10, 4, 4, 4) ROWS(1024, 16, 8, 1, 16, 16, 4)
+  ROWS(1280, 20, 8, 1, 10, 4, 4, 4, 2) ROWS(2048, 16, 8, 1, 16, 16, 8)
+  ROWS(1024, 16, 4, 1, 16, 16, 4) ROWS(1280, 20, 4, 1, 10, 4, 4, 4, 2)
+  ROWS(2048, 16, 4, 1, 16, 16, 8)
+#undef ROWS
+  return (int)hipErrorInvalidValue;
+}

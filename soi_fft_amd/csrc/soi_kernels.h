@@ -45,6 +45,13 @@ int soik_fft_col(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
 int soik_fft_row(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                  const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
                  hipStream_t st);
+int soik_fft_reg_supported(int L, int rows);
+int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
+                      const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
+                      hipStream_t st);
+int soik_fft_rows_reg(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
+                      const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
+                      hipStream_t st);
 int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st);
 int soik_compress(int *out, long stride, const double *x, int len, int nchunk, int e_max,
                   const int *emi, int *nwords, hipStream_t st);

@@ -1,11 +1,16 @@
-"""Benchmark: SOI FFT on one MI355X, BASELINE config 2 (N = 2^27 complex128, P = 32 virtual ranks,
-mu = 5/4, B = 72; input = 128 unit tones + Gaussian noise sigma 1e-3, seeded).
+"""Benchmark: plain FP64 SOI FFT on one MI355X at N = 2^27 complex128 (2 GiB), P = 32 virtual
+ranks, k = 1, mu = 5/4, B = 72; input = 128 unit-amplitude tones at random fractional bins plus
+complex Gaussian noise sigma 1e-3, seeded.
 
 One step = compute_soi over the device-resident N-vector (filter + S-DFT, segment FFT passes,
 demodulation).  compute_soi works in place, so step k transforms step k-1's output: the work of
-every step is identical.  Prints one JSON line.
+every step is identical.  Per-kernel durations come from HIP events that the library records on
+its stream around each kernel during the timed steps.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu-baseline]
+    python bench.py --copy-calibration ...   also runs one plain copy kernel (torch.neg over the
+                                             N-vector: 16N bytes read, 16N written) before the
+                                             timed steps, for calibrating HBM byte counters
 """
 import argparse
 import ctypes
@@ -31,11 +36,12 @@ def parse():
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--P", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log2n", type=int, default=23)
+    ap.add_argument("--cpu-sample-log2n", type=int, default=25)
+    ap.add_argument("--copy-calibration", action="store_true")
     return ap.parse_args()
 
 
-def config2_tones(N, seed=2027, n=128):
+def tones(N, seed=2027, n=128):
     import numpy as np
     rng = np.random.default_rng(seed)
     f = np.floor(rng.uniform(0, N, n)) + rng.uniform(0, 1, n)
@@ -59,6 +65,20 @@ def traffic_from_profiles(kernel_key):
     return None
 
 
+def fft_split(Mh):
+    """n1 x n2 split of the segment FFT as the library plans it for the register kernels: both
+    lengths in the supported set, n2 a multiple of 8, the larger factor as small as possible."""
+    lens = [256, 320, 512, 640, 1024, 1280, 2048, 2560]
+    best = None
+    for n1 in lens:
+        if Mh % n1:
+            continue
+        n2 = Mh // n1
+        if n2 in lens[:7] and n2 % 8 == 0 and (best is None or max(n1, n2) < max(best)):
+            best = (n1, n2)
+    return best if best else (Mh, 1)
+
+
 def cpu_baseline(log2n, P):
     """The oracle (C restatement of the reference, OpenMP) on this host's cores, on a bounded
     sample: N = 2^log2n with the same S, mu, B as the GPU workload."""
@@ -80,7 +100,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
-                                   "sharding is a design only, see DESIGN.md)"}))
+                                   "sharding is a design only)"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -88,7 +108,7 @@ def main():
 
     N, P = 1 << args.log2n, args.P
     dev = torch.device("cuda:0")
-    f, ph = config2_tones(N)
+    f, ph = tones(N)
     x = torch.empty(N, dtype=torch.complex128, device=dev)
     dp = ctypes.POINTER(ctypes.c_double)
     rc = soi.lib().soi_fill_tones(x.data_ptr(), N, f.ctypes.data_as(dp), ph.ctypes.data_as(dp),
@@ -103,12 +123,16 @@ def main():
     torch.cuda.synchronize()
     soi.compute_soi(d, y.data_ptr())
     rel_l2 = (torch.linalg.norm(y - ref) / torch.linalg.norm(ref)).item()
+    if args.copy_calibration:
+        torch.neg(x, out=y)
+        torch.cuda.synchronize()
     del ref, y
     torch.cuda.empty_cache()
 
     for _ in range(args.warmup):
         soi.compute_soi_async(d, x.data_ptr(), None)
     torch.cuda.synchronize()
+    soi.enable_timing(d, True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -121,35 +145,37 @@ def main():
     ms = e0.elapsed_time(e1) / args.steps
     flop = 5.0 * N * math.log2(N)
     gflops = flop / (ms * 1e-3) / 1e9
-
-    # per-kernel durations (HIP events on the descriptor's stream around each kernel)
-    soi.enable_timing(d, True)
-    acc = [0.0, 0.0, 0.0, 0.0]
-    nrep = max(3, min(args.steps, 10))
-    for _ in range(nrep):
-        soi.compute_soi(d, x.data_ptr())
-        for i, v in enumerate(soi.stage_times(d)):
-            acc[i] += v / nrep
+    # mean per-kernel durations over the timed steps (the last 64 when steps > 64)
+    acc = soi.stage_times(d)
     soi.enable_timing(d, False)
     S, M, Mh = d.S, d.M, d.M_hat
     soi.free_soi_descriptor(d)
 
-    mu = 5 / 4
-    filt_ms = acc[0]
-    filt_bytes = 16.0 * N * (1 + mu)                       # read alpha, write alpha_tilde
-    filt_flops = 8.0 * 72 * mu * N + 5.0 * mu * N * math.log2(S)
-    fb = filt_bytes / (filt_ms * 1e-3) / 1e9
-    ff = filt_flops / (filt_ms * 1e-3) / 1e12
-    hbm_frac, mfma_frac = fb / HBM_PEAK_GBS, ff / FP64_PEAK_TFLOPS
-    roofline = {"kernel": "k_filter_sdft_f64", "bound": "hbm" if hbm_frac >= mfma_frac else "mfma",
-                "achieved": round(fb if hbm_frac >= mfma_frac else ff, 2),
-                "peak": HBM_PEAK_GBS if hbm_frac >= mfma_frac else FP64_PEAK_TFLOPS,
-                "unit": "GB/s" if hbm_frac >= mfma_frac else "TFLOP/s",
-                "frac": round(max(hbm_frac, mfma_frac), 4),
-                "traffic": traffic_from_profiles("k_filter_sdft_f64 config2"),
-                "launch_ms": round(filt_ms, 4),
-                "algorithmic_bytes": filt_bytes, "algorithmic_flops": filt_flops,
-                "fp64_frac": round(mfma_frac, 4), "hbm_frac": round(hbm_frac, 4)}
+    mu, B = 5 / 4, 72
+    n1, n2 = fft_split(Mh)
+    # algorithmic bytes and flops per launch of each kernel (one launch covers all S segments)
+    kernels = {
+        "k_filter": (acc[0], 16.0 * N * (1 + mu),
+                     (4.0 * B + 6 + 5 * math.log2(S)) * mu * N),
+        "k_fft_cols": (acc[2], 32.0 * mu * N, (5.0 * math.log2(n1) + 6) * mu * N),
+        "k_fft_rows": (acc[3], 16.0 * mu * N + 16.0 * N + 16.0 * M,
+                       5.0 * math.log2(n2) * mu * N + 6.0 * N),
+    }
+    name = max(kernels, key=lambda k: kernels[k][0])
+    kms, kbytes, kflops = kernels[name]
+    hbm_frac = kbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    fp64_frac = kflops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS
+    hbm = hbm_frac >= fp64_frac
+    roofline = {"bound": "hbm" if hbm else "mfma",
+                "achieved": round(kbytes / (kms * 1e-3) / 1e9 if hbm
+                                  else kflops / (kms * 1e-3) / 1e12, 2),
+                "peak": HBM_PEAK_GBS if hbm else FP64_PEAK_TFLOPS,
+                "unit": "GB/s" if hbm else "TFLOP/s",
+                "frac": round(max(hbm_frac, fp64_frac), 4),
+                "traffic": traffic_from_profiles(f"{name} N=2^{args.log2n} P={P}"),
+                "kernel": name, "launch_ms": round(kms, 4),
+                "algorithmic_bytes": kbytes, "algorithmic_flops": kflops,
+                "hbm_frac": round(hbm_frac, 4), "fp64_frac": round(fp64_frac, 4)}
     stages = {"filter_sdft_ms": round(acc[0], 4), "vlc_ms": round(acc[1], 4),
               "fft_pass1_ms": round(acc[2], 4), "fft_pass2_demod_ms": round(acc[3], 4)}
     whole_bytes = 16.0 * N * (2 + 2 * mu)
@@ -159,14 +185,18 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: 128 unit-amplitude tones at random fractional bins + complex "
                 "Gaussian noise sigma=1e-3 per component, seeded",
-        "config": {"workload": "BASELINE config 2: plain SOI FP64, N=2^27 complex128, P=32 "
-                               "virtual ranks, k=1, mu=5/4, B=72",
+        "config": {"workload": f"plain SOI FP64, N=2^{args.log2n} complex128, P={P} virtual "
+                               "ranks, k=1, mu=5/4, B=72",
                    "N": N, "P": P, "k": 1, "S": S, "M": M, "M_hat": Mh, "n_mu": 5, "d_mu": 4,
-                   "B": 72, "form": "fp64"},
+                   "B": B, "form": "fp64", "fft_split": [n1, n2]},
         "rel_l2_vs_rocfft_fp64": rel_l2,
         "wall_s_timed": round(wall, 4),
         "whole_transform_hbm_roofline_frac": round(whole_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "stages": stages,
+        "kernel_rooflines": {k: {"ms": round(v[0], 4),
+                                 "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1),
+                                 "TFLOP/s": round(v[2] / (v[0] * 1e-3) / 1e12, 2)}
+                             for k, v in kernels.items()},
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -142,8 +142,11 @@ void compute_soi_async(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream);
 /* Stream the descriptor's kernels run on (hipStream_t as void*). */
 void *soi_stream(soi_desc_t *d);
 
-/* Per-stage times in milliseconds of the last compute_soi (filter, vlc, fft pass 1,
- * fft pass 2 + demodulation), measured with HIP events when timing was enabled. */
+/* Per-stage GPU times in milliseconds (filter + S-point DFT, vlc, fft pass 1, fft pass 2 +
+ * demodulation), from HIP events recorded on the stream each call runs on.  Enabling timing
+ * resets the record; soi_stage_times waits for the last recorded call and returns the mean over
+ * the last min(calls, 64) calls since then (the filter stage only after
+ * parallel_filter_subsampling).  Returns the number of stages written, 0 if none was recorded. */
 void soi_enable_timing(soi_desc_t *d, int on);
 int soi_stage_times(soi_desc_t *d, double *ms, int max_stages);
 

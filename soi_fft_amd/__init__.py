@@ -169,12 +169,17 @@ def stream_of(d):
 
 
 def enable_timing(d, on=True):
+    """Start (and reset) or stop recording per-stage HIP event times of each call."""
     lib().soi_enable_timing(ctypes.byref(d), int(on))
+    _check()
 
 
 def stage_times(d):
+    """Mean per-stage milliseconds [filter+S-DFT, vlc, fft pass 1, fft pass 2 + demodulation]
+    over the recorded calls (the last 64); [] if none was recorded."""
     buf = (ctypes.c_double * 4)()
     n = lib().soi_stage_times(ctypes.byref(d), buf, 4)
+    _check()
     return list(buf[:n])
 
 

@@ -1,6 +1,7 @@
 """Build libsoi_hip.so in-tree: HIP kernels (hipcc, gfx950) + C host layer (gcc).
 
     python -m soi_fft_amd.build          # or soi_fft_amd.build.build()
+    python -m soi_fft_amd.build --incremental   # recompile only sources newer than their objects
 
 Outputs soi_fft_amd/libsoi_hip.so; objects go to soi_fft_amd/_build/.
 """
@@ -39,7 +40,18 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def _stale(src, obj):
+    """True when obj is missing or older than src or any header under csrc/ and include/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith((".h", ".hpp"))]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    deps += [os.path.join(inc, h) for h in os.listdir(inc) if h.endswith(".h")]
+    return any(os.path.getmtime(dp) > t for dp in deps)
+
+
+def build(verbose=False, incremental=False):
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     cc = shutil.which("gcc") or "cc"
     os.makedirs(BUILD, exist_ok=True)
@@ -48,10 +60,12 @@ def build(verbose=False):
     for src in HIP_SOURCES:  # the HIP translation units compile in parallel
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, src + ".o")
+        objs.append(obj)
+        if incremental and not _stale(path, obj):
+            continue
         cmd = [hipcc] + HIPCC_FLAGS + ["-c", path, "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                                             text=True)))
-        objs.append(obj)
     for cmd, p in procs:
         out = p.communicate()[0]
         if p.returncode != 0:
@@ -71,4 +85,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv[1:])

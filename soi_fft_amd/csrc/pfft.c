@@ -70,6 +70,7 @@ int soi_last_status(void) { return g_status; }
 /* plan                                                                                        */
 /* ------------------------------------------------------------------------------------------ */
 #define NSTAGE 4
+#define NTSLOT 64 /* timed calls kept for soi_stage_times */
 typedef struct {
   int device;
   hipStream_t stream;
@@ -91,10 +92,11 @@ typedef struct {
   int *d_exps, *d_emi, *d_nwords;
   int *h_exps, *seg_max, *send_counts, total_max;
   int vlc_valid;
-  /* timing */
+  /* timing: NSTAGE+1 events per call, in a ring of NTSLOT calls */
   int timing;
-  hipEvent_t ev[NSTAGE + 1];
-  double stage_ms[NSTAGE];
+  long ncalls;          /* calls recorded since timing was enabled */
+  int filter_only_last;
+  hipEvent_t ev[NTSLOT][NSTAGE + 1];
 } soi_plan_t;
 
 static int is_smooth(long n)
@@ -281,8 +283,9 @@ static void free_plan(soi_plan_t *pl)
   free(pl->h_exps);
   free(pl->seg_max);
   free(pl->send_counts);
-  for (int e = 0; e <= NSTAGE; ++e)
-    if (pl->ev[e]) (void)hipEventDestroy(pl->ev[e]);
+  for (int c = 0; c < NTSLOT; ++c)
+    for (int e = 0; e <= NSTAGE; ++e)
+      if (pl->ev[c][e]) (void)hipEventDestroy(pl->ev[c][e]);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
   free(pl);
 }
@@ -412,7 +415,6 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       pl->seg_max = (int *)malloc(S * sizeof(int));
       pl->send_counts = (int *)malloc(S * sizeof(int));
     }
-    for (int e = 0; e <= NSTAGE; ++e) HIPCHK(hipEventCreate(&pl->ev[e]));
     HIPCHK(hipStreamSynchronize(pl->stream));
   }
   d->plan = pl;
@@ -474,9 +476,9 @@ static int is_device_ptr(const void *p)
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
-static void mark(soi_plan_t *pl, int e)
+static void mark(soi_plan_t *pl, int e, hipStream_t st)
 {
-  if (pl->timing) (void)hipEventRecord(pl->ev[e], pl->stream);
+  if (pl->timing) (void)hipEventRecord(pl->ev[pl->ncalls % NTSLOT][e], st);
 }
 
 /* pfft.c:390-440 restated: per-segment send counts and cost-balanced segment boundaries */
@@ -564,20 +566,20 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, soi_c64_t *X, hipStream_t st)
 {
   soi_c64_t *at = (soi_c64_t *)d->alpha_tilde;
   if (pl->fft_single) {
-    mark(pl, 2);
+    mark(pl, 2, st);
     KCHK(soik_fft_single(at, X, pl->tw1, (const soi_c64_t *)d->W_inv, pl->Mh, pl->M, (int)pl->S,
                          pl->single_count, st));
-    mark(pl, 3);
+    mark(pl, 3, st);
   } else if (pl->fft_reg) {
     KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
                            (int)pl->S, pl->Tc, st));
-    mark(pl, 3);
+    mark(pl, 3, st);
     KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Mh,
                            pl->M, (int)pl->S, pl->Tr, st));
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
                       (int)pl->S, st));
-    mark(pl, 3);
+    mark(pl, 3, st);
     KCHK(soik_fft_row(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Tr,
                       pl->Mh, pl->M, (int)pl->S, st));
   }
@@ -618,14 +620,18 @@ static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, 
       buf = pl->io;
     }
     pl->vlc_valid = 0;
-    mark(pl, 0);
+    mark(pl, 0, st);
     if (run_filter(d, pl, buf, st)) return;
-    mark(pl, 1);
+    mark(pl, 1, st);
     if (!filter_only) {
       if (d->use_vlc && run_vlc(d, pl, st)) return;
-      if (!pl->fft_single) mark(pl, 2);
+      if (!pl->fft_single) mark(pl, 2, st);
       if (run_fft(d, pl, buf, st)) return;
-      mark(pl, 4);
+      mark(pl, 4, st);
+    }
+    if (pl->timing) {
+      pl->filter_only_last = filter_only;
+      pl->ncalls++;
     }
     if (!dev) {
       if (!filter_only)
@@ -633,16 +639,6 @@ static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, 
       sync = 1;
     }
     if (sync) HIPCHK(hipStreamSynchronize(st));
-    if (pl->timing && sync) {
-      for (int e = 0; e < NSTAGE; ++e) pl->stage_ms[e] = 0;
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]) == hipSuccess) pl->stage_ms[0] = ms;
-      if (!filter_only) {
-        if (hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]) == hipSuccess) pl->stage_ms[1] = ms;
-        if (hipEventElapsedTime(&ms, pl->ev[2], pl->ev[3]) == hipSuccess) pl->stage_ms[2] = ms;
-        if (hipEventElapsedTime(&ms, pl->ev[3], pl->ev[4]) == hipSuccess) pl->stage_ms[3] = ms;
-      }
-    }
   }
   return;
 fail:
@@ -669,17 +665,47 @@ void *soi_stream(soi_desc_t *d)
 
 void soi_enable_timing(soi_desc_t *d, int on)
 {
-  soi_plan_t *pl = (soi_plan_t *)d->plan;
-  if (pl) pl->timing = on;
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised (call init_soi_descriptor)");
+    return;
+  }
+  if (on && !pl->ev[0][0]) {
+    HIPCHK(hipSetDevice(pl->device));
+    for (int c = 0; c < NTSLOT; ++c)
+      for (int e = 0; e <= NSTAGE; ++e) HIPCHK(hipEventCreate(&pl->ev[c][e]));
+  }
+  pl->timing = on;
+  pl->ncalls = 0;
+fail:
+  return;
 }
 
 int soi_stage_times(soi_desc_t *d, double *ms, int max_stages)
 {
-  soi_plan_t *pl = (soi_plan_t *)d->plan;
-  if (!pl) return 0;
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl || !ms || pl->ncalls == 0) return 0;
   int n = max_stages < NSTAGE ? max_stages : NSTAGE;
-  for (int e = 0; e < n; ++e) ms[e] = pl->stage_ms[e];
+  if (pl->filter_only_last && n > 1) n = 1;
+  long first = pl->ncalls > NTSLOT ? pl->ncalls - NTSLOT : 0;
+  int last_ev = pl->filter_only_last ? 1 : NSTAGE;
+  HIPCHK(hipSetDevice(pl->device));
+  HIPCHK(hipEventSynchronize(pl->ev[(pl->ncalls - 1) % NTSLOT][last_ev]));
+  for (int e = 0; e < n; ++e) ms[e] = 0;
+  for (long c = first; c < pl->ncalls; ++c) {
+    hipEvent_t *ev = pl->ev[c % NTSLOT];
+    for (int e = 0; e < n; ++e) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ev[e], ev[e + 1]));
+      ms[e] += t;
+    }
+  }
+  for (int e = 0; e < n; ++e) ms[e] /= (double)(pl->ncalls - first);
   return n;
+fail:
+  return 0;
 }
 
 int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts)

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the soi.h C ABI) with the oracle, rocFFT and closed forms.
 
 Tolerance: relative L2 <= 1e-12 against the oracle (the FP64 bar of the plain form).  At N = 2^27
-the oracle is too slow, so the full-size checks use rocFFT's FP64 transform
+(the 2 GiB workload) the oracle is too slow, so the full-size checks use rocFFT's FP64 transform
 (torch.fft.fft) and the exact DFT of the tone input at sampled bins."""
 import ctypes
 import sys
@@ -40,7 +40,7 @@ SHAPES = [
     (2 ** 13, 4, 1, {}),
     (2 ** 16, 8, 1, {}),
     (2 ** 16, 4, 2, {}),
-    (2 ** 20, 8, 1, {}),                       # BASELINE config 1 shape
+    (2 ** 20, 8, 1, {}),                       # the 8-rank CPU-runnable shape
     (3 * 2 ** 14, 4, 1, {}),
     (5 * 2 ** 14, 8, 1, {}),
     (2 ** 18, 16, 1, {}),
@@ -167,18 +167,18 @@ def test_device_generator_matches_oracle(soi, orc, torch_cuda):
     assert np.array_equal(xt.cpu().numpy(), orc.uniform_input(N, 77))
 
 
-def _config2_tones(seed, n=128, N=2 ** 27):
+def _tones_2e27(seed, n=128, N=2 ** 27):
     rng = np.random.default_rng(seed)
     f = np.floor(rng.uniform(0, N, n)) + rng.uniform(0, 1, n)
     ph = rng.uniform(0, 1, n)
     return f, ph
 
 
-def test_config2_full_size_vs_rocfft(soi, torch_cuda):
-    """BASELINE config 2: N = 2^27, P = 32, 128 unit tones + Gaussian noise (sigma 1e-3)."""
+def test_n2e27_full_size_vs_rocfft(soi, torch_cuda):
+    """N = 2^27, P = 32, 128 unit tones + Gaussian noise (sigma 1e-3)."""
     torch = torch_cuda
     N, P = 2 ** 27, 32
-    f, ph = _config2_tones(2027)
+    f, ph = _tones_2e27(2027)
     xt = torch.empty(N, dtype=torch.complex128, device="cuda")
     rc = soi.lib().soi_fill_tones(xt.data_ptr(), N, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                   ph.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(f), 1.0,
@@ -193,17 +193,17 @@ def test_config2_full_size_vs_rocfft(soi, torch_cuda):
     finally:
         soi.free_soi_descriptor(d)
     err = (torch.linalg.norm(xt - ref) / torch.linalg.norm(ref)).item()
-    print("config2 rel-L2 vs rocFFT", err, flush=True)
+    print("N=2^27 rel-L2 vs rocFFT", err, flush=True)
     assert err < TOL
     # Parseval: sum |X|^2 = N sum |x|^2
     assert abs(torch.sum(torch.abs(xt) ** 2).item() / (N * e_in) - 1) < 1e-12
 
 
-def test_config2_tones_closed_form(soi, torch_cuda):
+def test_tones_2e27_closed_form(soi, torch_cuda):
     """Noise-free tones at N = 2^27: exact DFT at every tone bin and 4096 random bins."""
     torch = torch_cuda
     N, P = 2 ** 27, 32
-    f, ph = _config2_tones(7)
+    f, ph = _tones_2e27(7)
     amps = np.ones(len(f))
     xt = torch.empty(N, dtype=torch.complex128, device="cuda")
     rc = soi.lib().soi_fill_tones(xt.data_ptr(), N, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -243,3 +243,65 @@ def test_window_modes(soi, orc, torch_cuda, N, P, kw, mode):
     finally:
         orc.set_window_mode(0)
     assert rel(out, ref) < max(2e-14, parity_tol(orc, p) / 10)
+
+
+def test_stage_timing_over_calls(soi, orc, torch_cuda):
+    """soi_enable_timing / soi_stage_times: per-stage event times recorded on the caller's stream,
+    averaged over the calls since timing was enabled; the output is unchanged by timing."""
+    torch = torch_cuda
+    N, P = 2 ** 20, 8
+    x = orc.uniform_input(N, 8)
+    d = soi.make_descriptor(N, P)
+    try:
+        assert soi.stage_times(d) == []          # nothing recorded yet
+        s = torch.cuda.Stream()
+        xt = to_dev(torch, x)
+        torch.cuda.synchronize()
+        soi.enable_timing(d, True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record(s)
+            soi.compute_soi_async(d, xt.data_ptr(), s.cuda_stream)
+            e1.record(s)
+        ms = soi.stage_times(d)
+        s.synchronize()
+        total = e0.elapsed_time(e1)
+        assert len(ms) == 4 and ms[0] > 0 and ms[2] > 0 and ms[3] > 0 and ms[1] >= 0
+        assert sum(ms) <= total * 1.05 + 0.01
+        got = xt.cpu().numpy()
+        for _ in range(3):                       # mean over several calls stays positive
+            soi.compute_soi_async(d, xt.data_ptr(), None)
+        assert all(v >= 0 for v in soi.stage_times(d))
+        soi.enable_timing(d, False)
+    finally:
+        soi.free_soi_descriptor(d)
+    ref, _ = orc.compute_soi(orc.params(N, P), x)
+    assert rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("tc,tr", [(4, 4), (4, 8), (8, 4)])
+def test_fft_tile_width_variants(soi, torch_cuda, monkeypatch, tc, tr):
+    """The 4-wide tile variants of the register FFT passes (SOI_FFT_TC / SOI_FFT_TR) compute the
+    same transform as the default 8-wide tiles: M_hat = 1280 x 2048 at N = 2^24, P = 8."""
+    torch = torch_cuda
+    N, P = 2 ** 24, 8
+    xt = torch.empty(N, dtype=torch.complex128, device="cuda")
+    assert soi.lib().soi_fill_uniform(xt.data_ptr(), N, 9, None) == 0
+    ref = torch.fft.fft(xt)
+    outs = []
+    for env in ({}, {"SOI_FFT_TC": str(tc), "SOI_FFT_TR": str(tr)}):
+        for kk in ("SOI_FFT_TC", "SOI_FFT_TR"):
+            monkeypatch.delenv(kk, raising=False)
+        for kk, v in env.items():
+            monkeypatch.setenv(kk, v)
+        y = xt.clone()
+        d = soi.make_descriptor(N, P)
+        try:
+            torch.cuda.synchronize()
+            soi.compute_soi(d, y.data_ptr())
+        finally:
+            soi.free_soi_descriptor(d)
+        outs.append(y)
+    err = (torch.linalg.norm(outs[1] - outs[0]) / torch.linalg.norm(outs[0])).item()
+    assert err < 1e-15
+    assert (torch.linalg.norm(outs[1] - ref) / torch.linalg.norm(ref)).item() < 3e-14

@@ -99,7 +99,7 @@ def test_batched_device_codec(soi, orc, torch_cuda):
 
 
 def _banded(N, S, seed, ntones=256):
-    """Tones inside one band of width N/S plus white noise at -60 dB."""
+    """Tones inside one band of width N/S plus white noise at -60 dB (the banded VLC workload)."""
     rng = np.random.default_rng(seed)
     lo = rng.integers(0, S) * (N // S)
     f = lo + rng.uniform(0, N // S, ntones)
@@ -116,7 +116,7 @@ def _banded(N, S, seed, ntones=256):
 @pytest.mark.parametrize("N,P,kw", [
     (2 ** 16, 8, {}),
     (7 * 2 ** 15, 16, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),
-    (7 * 2 ** 17, 64, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),   # config 5 at N/64
+    (7 * 2 ** 17, 64, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),   # banded VLC workload at N/64
 ])
 def test_soi_vlc_matches_oracle(soi, orc, torch_cuda, N, P, kw):
     torch = torch_cuda

@@ -83,7 +83,7 @@ typedef struct {
   soi_c64_t *post, *twS;
   /* fft */
   int fft_single, single_count, fft_reg;
-  int n1, n2, Tc, Tr, B1, lb1;
+  int n1, n2, Tc, Tr, B1, lb1, clay;
   soi_c64_t *tw1, *tw2, *thi, *tlo;
   /* buffers */
   soi_c64_t *io;        /* staging for host-pointer inputs */
@@ -222,6 +222,10 @@ static int plan_fft(soi_plan_t *pl)
       if (ec && (atoi(ec) == 4 || atoi(ec) == 8)) pl->Tc = atoi(ec);
       if (er && (atoi(er) == 4 || atoi(er) == 8)) pl->Tr = atoi(er);
       if (pl->n2 % pl->Tc || pl->n1 % pl->Tr) pl->Tc = pl->Tr = 8;
+      /* column-pass LDS layout: 0 = sequence index fastest, 1 = one run per sequence (tuning
+       * override SOI_FFT_CLAY, 8-wide tiles of 2048 or 2560 only) */
+      const char *el = getenv("SOI_FFT_CLAY");
+      pl->clay = (el && atoi(el) == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) ? 1 : 0;
       int lb = 0;
       while ((1L << lb) * (1L << lb) < Mh) ++lb;
       pl->B1 = 1 << lb;
@@ -572,7 +576,7 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, soi_c64_t *X, hipStream_t st)
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
     KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
-                           (int)pl->S, pl->Tc, st));
+                           (int)pl->S, pl->Tc, pl->clay, st));
     mark(pl, 3, st);
     KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Mh,
                            pl->M, (int)pl->S, pl->Tr, st));

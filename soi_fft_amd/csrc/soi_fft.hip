@@ -105,14 +105,29 @@ template <int... Rs> struct Radices {
   }
 };
 
-// LDS index of position pos of sequence q (q fastest), one pad slot per 16 positions
-template <int T> __device__ __forceinline__ int lidx(int pos, int q) { return (pos + (pos >> 4)) * T + q; }
+// LDS layouts of the T/NQP sequences exchanged at once (one double per slot; real and imaginary
+// halves go through in turn).  Both pad one slot per 16 positions.
+//   QFast:    sequence index fastest, (pos + pos/16)*TG + q -- suits waves that span the T
+//             sequences at a few positions (the column pass, whose lanes run over q first)
+//   SeqMajor: one padded run per sequence, q*LP + pos + pos/16 with LP = 2 (mod 16) -- suits
+//             waves that hold consecutive positions of one sequence (the row pass); the LP offset
+//             keeps the T-sequence transpose reads of the row pass on distinct banks
+template <int L, int TG> struct QFast {
+  static constexpr int size = (L + L / 16) * TG;
+  static __device__ __forceinline__ int at(int pos, int q) { return (pos + (pos >> 4)) * TG + q; }
+};
+template <int L, int TG> struct SeqMajor {
+  static constexpr int LP0 = L + L / 16;
+  static constexpr int LP = LP0 + ((2 - LP0 % 16) + 16) % 16;
+  static constexpr int size = LP * TG;
+  static __device__ __forceinline__ int at(int pos, int q) { return q * LP + pos + (pos >> 4); }
+};
 
 // Exchange a thread's stage outputs through LDS into the next stage's input order.  Outputs of
 // stage s (radix R, span Ns) for butterfly j land at (j - j%Ns)*R + j%Ns + k*Ns; stage s+1
 // (radix R2) butterfly j2 reads j2 + k*L/R2.  Done in parts: real then imaginary halves, and
 // NQP groups of the T sequences, so the LDS footprint is L*(T/NQP)*8 bytes (plus padding).
-template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2>
+template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2, typename LY>
 __device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, int ts)
 {
   constexpr int NB = E / R, NB2 = E / R2, TG = T / NQP;
@@ -129,7 +144,7 @@ __device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, in
           const int base = (j - jm) * R + jm;
 #pragma unroll
           for (int k = 0; k < R; ++k)
-            lds[lidx<TG>(base + k * NS, ql)] = half ? v[c * R + k].y : v[c * R + k].x;
+            lds[LY::at(base + k * NS, ql)] = half ? v[c * R + k].y : v[c * R + k].x;
         }
       }
       __syncthreads();
@@ -139,7 +154,7 @@ __device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, in
           const int j = ts + TS * c;
 #pragma unroll
           for (int k = 0; k < R2; ++k) {
-            const double val = lds[lidx<TG>(j + k * (L / R2), ql)];
+            const double val = lds[LY::at(j + k * (L / R2), ql)];
             if (half) v[c * R2 + k].y = val; else v[c * R2 + k].x = val;
           }
         }
@@ -168,7 +183,7 @@ __device__ __forceinline__ void stage(double2 (&v)[E], int ts, const double2 *__
 }
 
 // Run stages 0..n-1 on register data already loaded for stage 0.
-template <int L, int E, int T, int NQP, int TS, typename RS, int S0>
+template <int L, int E, int T, int NQP, int TS, typename RS, typename LY, int S0>
 __device__ __forceinline__ void run_stages(double2 (&v)[E], double *lds, int q, int ts,
                                            const double2 *__restrict__ tw)
 {
@@ -176,8 +191,8 @@ __device__ __forceinline__ void run_stages(double2 (&v)[E], double *lds, int q, 
   constexpr int NS = RS::span(S0);
   stage<L, E, TS, R, NS>(v, ts, tw);
   if constexpr (S0 + 1 < RS::n) {
-    exchange<L, E, T, NQP, TS, R, NS, RS::at(S0 + 1)>(v, lds, q, ts);
-    run_stages<L, E, T, NQP, TS, RS, S0 + 1>(v, lds, q, ts, tw);
+    exchange<L, E, T, NQP, TS, R, NS, RS::at(S0 + 1), LY>(v, lds, q, ts);
+    run_stages<L, E, T, NQP, TS, RS, LY, S0 + 1>(v, lds, q, ts, tw);
   }
 }
 
@@ -200,7 +215,7 @@ __device__ __forceinline__ int out_index(int ts, int c, int k)
 // block otherwise
 constexpr int fft_minw(int T, int threads) { return ((T == 4 ? 2 : 1) * threads / 64 + 3) / 4; }
 
-template <int L, int E, int T, int NQP, typename RS>
+template <int L, int E, int T, int NQP, typename RS, typename LY>
 __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_cols(double2 *__restrict__ data,
                                                           const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ thi,
@@ -226,7 +241,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_c
 #pragma unroll
     for (int k = 0; k < R0; ++k) v[c * R0 + k] = seg[(unsigned)(ts + TS * c + k * (L / R0)) * (unsigned)n2s];
 
-  run_stages<L, E, T, NQP, TS, RS, 0>(v, ldsd, q, ts, tw);
+  run_stages<L, E, T, NQP, TS, RS, LY, 0>(v, ldsd, q, ts, tw);
 
   constexpr int RL = RS::at(RS::n - 1);
 #pragma unroll
@@ -270,7 +285,8 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_r
 #pragma unroll
     for (int k = 0; k < R0; ++k) v[c * R0 + k] = row[ts + TS * c + k * (L / R0)];
 
-  run_stages<L, E, T, NQP, TS, RS, 0>(v, ldsd, q, ts, tw);
+  using LY = SeqMajor<L, T / NQP>;
+  run_stages<L, E, T, NQP, TS, RS, LY, 0>(v, ldsd, q, ts, tw);
 
   // transpose through LDS so that stores run over the T rows (m = a + q + n1*m2), in NQP groups
   // of T/NQP rows and real/imaginary halves
@@ -290,13 +306,13 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_r
         for (int c = 0; c < E / RL; ++c)
 #pragma unroll
           for (int k = 0; k < RL; ++k)
-            ldsd[lidx<TG>(out_index<L, TS, RS>(ts, c, k), ql)] = half ? v[c * RL + k].y : v[c * RL + k].x;
+            ldsd[LY::at(out_index<L, TS, RS>(ts, c, k), ql)] = half ? v[c * RL + k].y : v[c * RL + k].x;
       }
       __syncthreads();
 #pragma unroll
       for (int c = 0; c < EG; ++c) {
         const int e = tid + c * NT;  // e = m2*TG + qq
-        const double val = ldsd[lidx<TG>(e / TG, e % TG)];
+        const double val = ldsd[LY::at(e / TG, e % TG)];
         if (half) w[c].y = val; else w[c].x = val;
       }
       __syncthreads();
@@ -313,14 +329,12 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_r
 // ---------------------------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------------------------
-template <int L, int T, int NQP> constexpr size_t fft_lds_bytes() { return (size_t)(L + L / 16) * (T / NQP) * 8; }
-
-template <int L, int E, int T, int NQP, typename RS>
+template <int L, int E, int T, int NQP, typename RS, typename LY>
 static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, const double2 *tlo,
                        int n2, int lb1, long Mh, int S, hipStream_t st)
 {
-  auto kern = k_fft_cols<L, E, T, NQP, RS>;
-  constexpr size_t lds = fft_lds_bytes<L, T, NQP>();
+  auto kern = k_fft_cols<L, E, T, NQP, RS, LY>;
+  constexpr size_t lds = (size_t)LY::size * 8;
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -337,7 +351,7 @@ static int launch_rows(const double2 *data, double2 *X, const double2 *tw, const
                        int n1, long Mh, long M, int S, hipStream_t st)
 {
   auto kern = k_fft_rows<L, E, T, NQP, RS>;
-  constexpr size_t lds = fft_lds_bytes<L, T, NQP>();
+  constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * 8;
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -369,18 +383,22 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 // (64-byte runs, two blocks per CU).
 extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
                                  const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                 int T, hipStream_t st)
+                                 int T, int layout, hipStream_t st)
 {
   if (n2 % T) return (int)hipErrorInvalidValue;
 #define COLS(LV, EV, TV, NQ, ...) \
-  if (n1 == LV && T == TV) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+  if (n1 == LV && T == TV && layout == 0) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>, QFast<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+#define COLS_SM(LV, EV, TV, NQ, ...) \
+  if (n1 == LV && T == TV && layout == 1) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>, SeqMajor<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
   COLS(256, 16, 8, 1, 16, 16) COLS(320, 20, 8, 1, 10, 4, 4, 2) COLS(512, 16, 8, 1, 16, 16, 2)
   COLS(640, 20, 8, 1, 10, 4, 4, 4) COLS(1024, 16, 8, 1, 16, 16, 4)
   COLS(1280, 20, 8, 1, 10, 4, 4, 4, 2) COLS(2048, 16, 8, 1, 16, 16, 8)
   COLS(2560, 20, 8, 2, 10, 4, 4, 4, 4)
   COLS(1024, 16, 4, 1, 16, 16, 4) COLS(1280, 20, 4, 1, 10, 4, 4, 4, 2)
   COLS(2048, 16, 4, 1, 16, 16, 8) COLS(2560, 20, 4, 2, 10, 4, 4, 4, 4)
+  COLS_SM(2048, 16, 8, 1, 16, 16, 8) COLS_SM(2560, 20, 8, 2, 10, 4, 4, 4, 4)
 #undef COLS
+#undef COLS_SM
   return (int)hipErrorInvalidValue;
 }
 

@@ -48,7 +48,7 @@ int soik_fft_row(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
 int soik_fft_reg_supported(int L, int rows);
 int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                       const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
-                      hipStream_t st);
+                      int layout, hipStream_t st);
 int soik_fft_rows_reg(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                       const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
                       hipStream_t st);

@@ -201,13 +201,17 @@ static int plan_fft(soi_plan_t *pl)
   {
     long best = LONG_MAX;
     int bn1 = 0;
+    /* SOI_FFT_N1 forces the column length (tuning) */
+    const char *en = getenv("SOI_FFT_N1");
+    long force = en ? atol(en) : 0;
     for (long n1 = 8; n1 <= 4096; n1 += 8) {
       if (Mh % n1) continue;
       long n2 = Mh / n1;
       if (n2 % 8 || n2 > 4096) continue;
       if (!soik_fft_reg_supported((int)n1, 0) || !soik_fft_reg_supported((int)n2, 1)) continue;
       long cost = (n1 > n2 ? n1 : n2);
-      if (cost < best) {
+      if (force == n1) cost = 0;
+      if (cost <= best) {   /* ties: the longer column */
         best = cost;
         bn1 = (int)n1;
       }
@@ -222,10 +226,19 @@ static int plan_fft(soi_plan_t *pl)
       if (ec && (atoi(ec) == 4 || atoi(ec) == 8)) pl->Tc = atoi(ec);
       if (er && (atoi(er) == 4 || atoi(er) == 8)) pl->Tr = atoi(er);
       if (pl->n2 % pl->Tc || pl->n1 % pl->Tr) pl->Tc = pl->Tr = 8;
-      /* column-pass LDS layout: 0 = sequence index fastest, 1 = one run per sequence (tuning
-       * override SOI_FFT_CLAY, 8-wide tiles of 2048 or 2560 only) */
-      const char *el = getenv("SOI_FFT_CLAY");
-      pl->clay = (el && atoi(el) == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) ? 1 : 0;
+      if (pl->n2 == 2560) pl->Tr = 4; /* 2560-point rows exist with 4-wide tiles only */
+      /* column-pass kernel variant: 1 = one padded LDS run per sequence (8-wide tiles of 2048
+       * or 2560); 2 = one 4-wide block per CU with the whole register budget (2560, the
+       * default there: 20 values per thread do not fit 1024-thread blocks); SOI_FFT_CVAR
+       * overrides for tuning */
+      const char *el = getenv("SOI_FFT_CVAR");
+      int cv = el ? atoi(el) : (pl->n1 == 2560 ? 2 : 0);
+      pl->clay = 0;
+      if (cv == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) pl->clay = 1;
+      if (cv == 2 && pl->n1 == 2560) {
+        pl->clay = 2;
+        pl->Tc = 4;
+      }
       int lb = 0;
       while ((1L << lb) * (1L << lb) < Mh) ++lb;
       pl->B1 = 1 << lb;

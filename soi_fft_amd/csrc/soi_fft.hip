@@ -211,12 +211,11 @@ __device__ __forceinline__ int out_index(int ts, int c, int k)
 // ---------------------------------------------------------------------------------------------
 // Pass 1: T adjacent columns of length L = n1 (stride n2), twiddled, in place
 // ---------------------------------------------------------------------------------------------
-// waves per SIMD the register budget is sized for: two blocks per CU for the 4-wide tiles, one
-// block otherwise
-constexpr int fft_minw(int T, int threads) { return ((T == 4 ? 2 : 1) * threads / 64 + 3) / 4; }
+// waves per SIMD the register budget is sized for, with BPC blocks resident per CU
+constexpr int fft_minw(int BPC, int threads) { return (BPC * threads / 64 + 3) / 4; }
 
-template <int L, int E, int T, int NQP, typename RS, typename LY>
-__global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_cols(double2 *__restrict__ data,
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY>
+__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_cols(double2 *__restrict__ data,
                                                           const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ thi,
                                                           const double2 *__restrict__ tlo,
@@ -258,8 +257,8 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_c
 // ---------------------------------------------------------------------------------------------
 // Pass 2: T adjacent rows of length L = n2, demodulated, stored transposed with 128 B runs
 // ---------------------------------------------------------------------------------------------
-template <int L, int E, int T, int NQP, typename RS>
-__global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_rows(const double2 *__restrict__ data,
+template <int L, int E, int T, int NQP, int BPC, typename RS>
+__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_rows(const double2 *__restrict__ data,
                                                           double2 *__restrict__ X,
                                                           const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ winv,
@@ -329,11 +328,11 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(T, T * (L / E))) void k_fft_r
 // ---------------------------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------------------------
-template <int L, int E, int T, int NQP, typename RS, typename LY>
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY>
 static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, const double2 *tlo,
                        int n2, int lb1, long Mh, int S, hipStream_t st)
 {
-  auto kern = k_fft_cols<L, E, T, NQP, RS, LY>;
+  auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY>;
   constexpr size_t lds = (size_t)LY::size * 8;
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
@@ -346,11 +345,11 @@ static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, con
   return (int)hipGetLastError();
 }
 
-template <int L, int E, int T, int NQP, typename RS>
+template <int L, int E, int T, int NQP, int BPC, typename RS>
 static int launch_rows(const double2 *data, double2 *X, const double2 *tw, const double2 *winv,
                        int n1, long Mh, long M, int S, hipStream_t st)
 {
-  auto kern = k_fft_rows<L, E, T, NQP, RS>;
+  auto kern = k_fft_rows<L, E, T, NQP, BPC, RS>;
   constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * 8;
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
@@ -366,14 +365,15 @@ static int launch_rows(const double2 *data, double2 *X, const double2 *tw, const
 // Register-FFT lengths (E values per thread, radices in stage order):
 //   256 = 16.16, 512 = 16.16.2, 1024 = 16.16.4, 2048 = 16.16.8            (E = 16)
 //   320 = 10.4.4.2, 640 = 10.4.4.4, 1280 = 10.4.4.4.2, 2560 = 10.4.4.4.4   (E = 20)
-// Columns take any of them (2560 exchanges in two sequence groups); rows take up to 2048.
+// Columns take any of them (2560 exchanges in two sequence groups at 8-wide tiles); 2560-point
+// rows run 4-wide tiles.
 extern "C" int soik_fft_reg_supported(int L, int rows)
 {
   switch (L) {
     case 256: case 320: case 512: case 640: case 1024: case 1280: case 2048:
       return 1;
     case 2560:
-      return !rows;
+      return 1;
     default:
       return 0;
   }
@@ -381,24 +381,26 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 
 // Tile width T: 8 (128-byte runs, one 1024-thread block per CU for the long lengths) or 4
 // (64-byte runs, two blocks per CU).
+// Tile width T: 8 (128-byte runs) or 4 (64-byte runs).  variant selects a tuning alternative
+// for some lengths: 1 = one padded LDS run per sequence (2048, 2560 at T = 8); 2 = the whole
+// 256-VGPR budget for one 4-wide block per CU (2560 at T = 4).
 extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
                                  const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                 int T, int layout, hipStream_t st)
+                                 int T, int variant, hipStream_t st)
 {
   if (n2 % T) return (int)hipErrorInvalidValue;
-#define COLS(LV, EV, TV, NQ, ...) \
-  if (n1 == LV && T == TV && layout == 0) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>, QFast<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
-#define COLS_SM(LV, EV, TV, NQ, ...) \
-  if (n1 == LV && T == TV && layout == 1) return launch_cols<LV, EV, TV, NQ, Radices<__VA_ARGS__>, SeqMajor<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
-  COLS(256, 16, 8, 1, 16, 16) COLS(320, 20, 8, 1, 10, 4, 4, 2) COLS(512, 16, 8, 1, 16, 16, 2)
-  COLS(640, 20, 8, 1, 10, 4, 4, 4) COLS(1024, 16, 8, 1, 16, 16, 4)
-  COLS(1280, 20, 8, 1, 10, 4, 4, 4, 2) COLS(2048, 16, 8, 1, 16, 16, 8)
-  COLS(2560, 20, 8, 2, 10, 4, 4, 4, 4)
-  COLS(1024, 16, 4, 1, 16, 16, 4) COLS(1280, 20, 4, 1, 10, 4, 4, 4, 2)
-  COLS(2048, 16, 4, 1, 16, 16, 8) COLS(2560, 20, 4, 2, 10, 4, 4, 4, 4)
-  COLS_SM(2048, 16, 8, 1, 16, 16, 8) COLS_SM(2560, 20, 8, 2, 10, 4, 4, 4, 4)
+#define COLS(VV, LV, EV, TV, NQ, BPC, LY, ...) \
+  if (n1 == LV && T == TV && variant == VV) \
+    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+  COLS(0, 256, 16, 8, 1, 1, QFast, 16, 16) COLS(0, 320, 20, 8, 1, 1, QFast, 10, 4, 4, 2)
+  COLS(0, 512, 16, 8, 1, 1, QFast, 16, 16, 2) COLS(0, 640, 20, 8, 1, 1, QFast, 10, 4, 4, 4)
+  COLS(0, 1024, 16, 8, 1, 1, QFast, 16, 16, 4) COLS(0, 1280, 20, 8, 1, 1, QFast, 10, 4, 4, 4, 2)
+  COLS(0, 2048, 16, 8, 1, 1, QFast, 16, 16, 8) COLS(0, 2560, 20, 8, 2, 1, QFast, 10, 4, 4, 4, 4)
+  COLS(0, 1024, 16, 4, 1, 2, QFast, 16, 16, 4) COLS(0, 1280, 20, 4, 1, 2, QFast, 10, 4, 4, 4, 2)
+  COLS(0, 2048, 16, 4, 1, 2, QFast, 16, 16, 8) COLS(0, 2560, 20, 4, 2, 2, QFast, 10, 4, 4, 4, 4)
+  COLS(1, 2048, 16, 8, 1, 1, SeqMajor, 16, 16, 8) COLS(1, 2560, 20, 8, 2, 1, SeqMajor, 10, 4, 4, 4, 4)
+  COLS(2, 2560, 20, 4, 1, 1, QFast, 10, 4, 4, 4, 4)
 #undef COLS
-#undef COLS_SM
   return (int)hipErrorInvalidValue;
 }
 
@@ -407,13 +409,14 @@ extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 
                                  int T, hipStream_t st)
 {
   if (n1 % T) return (int)hipErrorInvalidValue;
-#define ROWS(LV, EV, TV, NQ, ...) \
-  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, Radices<__VA_ARGS__>>(data, X, tw2, winv, n1, Mh, M, S, st);
-  ROWS(256, 16, 8, 1, 16, 16) ROWS(320, 20, 8, 1, 10, 4, 4, 2) ROWS(512, 16, 8, 1, 16, 16, 2)
-  ROWS(640, 20, 8, 1, 10, 4, 4, 4) ROWS(1024, 16, 8, 1, 16, 16, 4)
-  ROWS(1280, 20, 8, 1, 10, 4, 4, 4, 2) ROWS(2048, 16, 8, 1, 16, 16, 8)
-  ROWS(1024, 16, 4, 1, 16, 16, 4) ROWS(1280, 20, 4, 1, 10, 4, 4, 4, 2)
-  ROWS(2048, 16, 4, 1, 16, 16, 8)
+#define ROWS(LV, EV, TV, NQ, BPC, ...) \
+  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>>(data, X, tw2, winv, n1, Mh, M, S, st);
+  ROWS(256, 16, 8, 1, 1, 16, 16) ROWS(320, 20, 8, 1, 1, 10, 4, 4, 2) ROWS(512, 16, 8, 1, 1, 16, 16, 2)
+  ROWS(640, 20, 8, 1, 1, 10, 4, 4, 4) ROWS(1024, 16, 8, 1, 1, 16, 16, 4)
+  ROWS(1280, 20, 8, 1, 1, 10, 4, 4, 4, 2) ROWS(2048, 16, 8, 1, 1, 16, 16, 8)
+  ROWS(2560, 20, 4, 1, 1, 10, 4, 4, 4, 4)
+  ROWS(1024, 16, 4, 1, 2, 16, 16, 4) ROWS(1280, 20, 4, 1, 2, 10, 4, 4, 4, 2)
+  ROWS(2048, 16, 4, 1, 2, 16, 16, 8)
 #undef ROWS
   return (int)hipErrorInvalidValue;
 }

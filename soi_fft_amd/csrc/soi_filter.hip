@@ -15,9 +15,10 @@
 // the e^{i pi t} factor splits into a per-(theta,i) phase and (-1)^b).  Complex-tap form
 // (CT = true): the reference's complex window values, 4 FMAs per tap.
 //
-// Per tile: direct-to-LDS loads (global_load_lds_dwordx4) of the contiguous input rows, the
-// fully unrolled FMA block, the tap-slice reduction through LDS, the post-phase multiply, an LDS
-// Stockham S-point DFT per row, and coalesced stores of n_mu*U*W consecutive R per segment.
+// Per tile: the fully unrolled FMA block over an LDS copy of the tile's contiguous input rows,
+// the tap-slice reduction through LDS, the post-phase multiply, an LDS Stockham S-point DFT per
+// row, and coalesced stores of n_mu*U*W consecutive R per segment.  The next tile's rows are
+// loaded into registers while this tile computes, and barriers wait on LDS traffic only.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "soi_common.hpp"
@@ -26,6 +27,7 @@
 using namespace soi;
 
 typedef __attribute__((address_space(3))) void lds_void_t;
+
 
 template <int BP, bool CT> struct Taps;
 template <int BP> struct Taps<BP, false> { double v[BP]; };
@@ -67,130 +69,260 @@ __device__ __forceinline__ void conv_rows(const double2 *__restrict__ col, const
 template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
 struct FilterCfg {
   static constexpr int T = S * NT * NP * W;          // threads
+  static constexpr int NWAVES = T / 64;
   static constexpr int TJ = U * W;                   // row groups per tile
   static constexpr int ROWS = TJ * DMU + NP * BP - DMU;  // input rows per tile
   static constexpr int XN = (ROWS * S + 63) / 64 * 64;   // staged complex values (whole waves)
+  static constexpr int NDMA = XN / 64;               // wave-wide 16-byte loads per tile
   static constexpr int GR = TJ * NT;                 // output rows per tile (this block's phases)
   static constexpr int GS = S + 1;                   // padded G row stride
   static constexpr int GN = GR * GS;
-  static constexpr int LDSN = (XN > GN ? XN : GN);   // complex values; + S twiddles after
+  static constexpr int BUFN = (XN > GN ? XN : GN);   // one tile buffer (G reuses it)
   static constexpr int NG = NMU / NT;                // phase groups
-  // register budget: two blocks per CU for blocks of up to 6 waves, else one
-  static constexpr int BLK_PER_CU = (T <= 384) ? 2 : 1;
-  static constexpr int MINW = (BLK_PER_CU * (T / 64) + 3) / 4;
+  // two tile buffers (the next tile's rows stream in while this one computes) when they fit
+  static constexpr bool DB = 2 * BUFN * 16 + S * 16 <= 160 * 1024;
+  static constexpr int LDS_BYTES = ((DB ? 2 : 1) * BUFN + S) * 16;
+  // blocks resident per CU (LDS and registers), and the register budget that follows
+  static constexpr int BPC = (LDS_BYTES <= 80 * 1024 && T <= 320) ? 2 : 1;
+  static constexpr int MINW = (BPC * NWAVES + 3) / 4;
+  static constexpr int NSC = (GR * S + T - 1) / T;   // stores per thread per tile (at most)
+  static constexpr int NSC_MIN = (GR * S) / T;       // (at least)
 };
 
 constexpr int filter_threads(int S, int NT, int NP, int W) { return S * NT * NP * W; }
-constexpr int filter_minw(int T) { return (((T <= 384) ? 2 : 1) * (T / 64) + 3) / 4; }
+
+// Barrier for LDS hand-off only: waits for this wave's LDS operations, not for its global
+// stores, which stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// s_waitcnt vmcnt(n) (expcnt and lgkmcnt left alone), through the builtin so the compiler's
+// own wait tracking sees it
+template <int n> __device__ __forceinline__ void wait_vmcnt()
+{
+  static_assert(n >= 0 && n < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14));
+}
+
+// In-place S-point DFT of each of the GR rows of G (row r at G + r*GS, unit stride), Stockham
+// passes of radix 8 then 4 / 2 with compile-time shapes; tw[m] = e^{-2 pi i m/S} in LDS.
+template <int S> constexpr int sdft_radix(int pass)
+{
+  int rem = S, p = 0, r = 0;
+  while (rem > 1) {
+    r = (rem % 8 == 0 && rem != 16) ? 8 : (rem % 4 == 0 ? 4 : 2);
+    if (p == pass) return r;
+    rem /= r;
+    ++p;
+  }
+  return 0;
+}
+
+template <int S, int GR, int GS, int T, int P, int NS>
+__device__ __forceinline__ void sdft_passes(double2 *G, const double2 *tw, int tid)
+{
+  constexpr int R = sdft_radix<S>(P);
+  if constexpr (R != 0) {
+    constexpr int NBF = S / R;                 // butterflies per row
+    constexpr int TOT = GR * NBF;
+    constexpr int NB = (TOT + T - 1) / T;
+    double2 v[NB][R];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int g = tid + c * T;
+      if (TOT % T == 0 || g < TOT) {
+        const int q = g / NBF, j = g % NBF;
+        const double2 *src = G + q * GS + j;
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[c][k] = src[k * NBF];
+      }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int g = tid + c * T;
+      if (TOT % T == 0 || g < TOT) {
+        const int q = g / NBF, j = g % NBF;
+        const int jm = j % NS;
+        if constexpr (NS > 1) {
+          constexpr int step = S / (NS * R);
+#pragma unroll
+          for (int k = 1; k < R; ++k) v[c][k] = cmul(v[c][k], tw[k * jm * step]);
+        }
+        dft_r<R>(v[c]);
+        double2 *dst = G + q * GS + (j - jm) * R + jm;
+#pragma unroll
+        for (int k = 0; k < R; ++k) dst[k * NS] = v[c][k];
+      }
+    }
+    lds_barrier();
+    sdft_passes<S, GR, GS, T, P + 1, NS * R>(G, tw, tid);
+  }
+}
+
+// Stage the XN input values of a tile (contiguous from element g0, wrapping once at N) into
+// dst with direct-to-LDS loads; wave w moves 64-value chunks w, w + NWAVES, ...
+template <typename C>
+__device__ __forceinline__ void dma_rows(double2 *dst, const double2 *__restrict__ x, long g0,
+                                         long N, int wave, int lane)
+{
+#pragma unroll
+  for (int c = wave; c < C::NDMA; c += C::NWAVES) {
+    long gi = g0 + c * 64 + lane;
+    if (gi >= N) gi -= N;
+    __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * 64), 16,
+                                     0, 0);
+  }
+}
+
+// Per-thread state that stays in registers for the whole kernel.
+template <int BP, bool CT> struct FilterThread {
+  Taps<BP, CT> h;
+  double2 pf;        // post phase
+  int i, tl, p, w, g;
+  int tid, wave, lane;
+};
+
+// One tile: prefetch the next tile's rows into nxt (when there is a next tile), FMA block from
+// cur, tap-slice reduction + post phase + S-point DFT in cur, stores.  Barriers wait on LDS
+// traffic only: the prefetch and the stores stay in flight.
+template <typename C, int S, int NMU, int NT, int DMU, int BP, int U, int NP, bool CT>
+__device__ __forceinline__ void filter_tile(const FilterThread<BP, CT> &th, double2 *cur,
+                                            double2 *nxt, const double2 *tw,
+                                            const double2 *__restrict__ x,
+                                            double2 *__restrict__ at, long tile, bool more,
+                                            long N, long Mh, long nJ)
+{
+  const long J0 = tile * C::TJ;
+  if constexpr (C::DB) {
+    if (more) dma_rows<C>(nxt, x, (tile + 1) * C::TJ * DMU * S, N, th.wave, th.lane);
+  } else {
+    dma_rows<C>(cur, x, tile * C::TJ * DMU * S, N, th.wave, th.lane);
+    wait_vmcnt<0>();
+    lds_barrier();
+  }
+
+  // ---- FMA block
+  double2 acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc[u] = double2{0.0, 0.0};
+  conv_rows<0, (U - 1) * DMU + BP, S, U, DMU, BP, CT>(cur + (th.w * U * DMU + th.p * BP) * S + th.i,
+                                                      th.h, acc);
+  lds_barrier();
+
+  // ---- tap-slice reduction (fixed order p = NP-1 .. 0) and post phase, into G = cur
+  double2 *G = cur;
+  const int gbase = (th.w * U) * NT + th.tl;
+#pragma unroll
+  for (int q = NP - 1; q >= 0; --q) {
+    if (th.p == q) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double2 a = acc[u];
+        double2 *gp = G + (gbase + u * NT) * C::GS + th.i;
+        if (q != NP - 1) a = cadd(a, *gp);
+        if (q == 0 && !CT) a = cmul(a, th.pf);
+        *gp = a;
+      }
+    }
+    lds_barrier();
+  }
+
+  // ---- S-point DFT of each row
+  sdft_passes<S, C::GR, C::GS, C::T, 0, 1>(G, tw, th.tid);
+
+  // ---- store segment-major: R = J*n_mu + theta (r fastest: runs of consecutive R)
+#pragma unroll
+  for (int c = 0; c < C::NSC; ++c) {
+    const int e = th.tid + c * C::T;
+    if ((C::GR * S) % C::T == 0 || e < C::GR * S) {
+      const int s = e / C::GR, r = e % C::GR;
+      const long J = J0 + r / NT;
+      if (J < nJ) {
+        const long R = J * NMU + th.g * NT + r % NT;
+        at[(long)s * Mh + R] = G[r * C::GS + s];
+      }
+    }
+  }
+  if constexpr (C::DB) {
+    // this wave's prefetch loads are complete once at most its stores are outstanding (the
+    // stores were issued after them; a tile with a successor stores every row)
+    asm volatile("" ::: "memory");
+    if (more) wait_vmcnt<C::NSC_MIN>();
+  }
+  lds_barrier();
+}
 
 template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
 __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
-                             filter_minw(filter_threads(S, NT, NP, W))) void k_filter(
+                             (FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT>::MINW)) void k_filter(
     const double2 *__restrict__ x, double2 *__restrict__ at, const double *__restrict__ taps,
     const double2 *__restrict__ post, const double2 *__restrict__ twS, long M, long Mh, long nJ,
     int tiles_per_block)
 {
   typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT> C;
-  extern __shared__ double2 lds[];
-  double2 *X = lds;
-  double2 *G = lds;  // reuses the input rows once the FMA block is done
-  double2 *tw = lds + C::LDSN;
-  const int tid = threadIdx.x;
-  const int i = tid % S;
-  int rest = tid / S;
-  const int tl = rest % NT;
+  __shared__ double2 bufA[C::BUFN];
+  __shared__ double2 bufB[C::DB ? C::BUFN : 1];
+  __shared__ double2 tw[S];
+  FilterThread<BP, CT> th;
+  th.tid = threadIdx.x;
+  th.wave = th.tid >> 6;
+  th.lane = th.tid & 63;
+  th.i = th.tid % S;
+  int rest = th.tid / S;
+  th.tl = rest % NT;
   rest /= NT;
-  const int p = rest % NP;
-  const int w = rest / NP;
-  const int lane = tid & 63, wave = tid >> 6;
-  constexpr int NWAVES = C::T / 64;
+  th.p = rest % NP;
+  th.w = rest / NP;
 
   // phase group: the slow index of the block id so blocks sharing input rows are 8-apart
   // (same XCD under round-robin dispatch) and walk the same tiles at the same time
   const int nblk_g = gridDim.x / C::NG;
-  const int g = blockIdx.x / nblk_g;
-  const int bg = blockIdx.x - g * nblk_g;
-  const int theta = g * NT + tl;
-
-  Taps<BP, CT> h;
+  th.g = blockIdx.x / nblk_g;
+  const int bg = blockIdx.x - th.g * nblk_g;
+  const int theta = th.g * NT + th.tl;
   {
-    const long base = ((long)theta * S + i) * (NP * BP) + p * BP;
+    const long base = ((long)theta * S + th.i) * (NP * BP) + th.p * BP;
     if constexpr (CT) {
       const double2 *tp = reinterpret_cast<const double2 *>(taps) + base;
 #pragma unroll
       for (int b = 0; b < BP; ++b) {
         double2 v = tp[b];
-        h.re[b] = v.x;
-        h.im[b] = v.y;
+        th.h.re[b] = v.x;
+        th.h.im[b] = v.y;
       }
     } else {
       const double *tp = taps + base;
 #pragma unroll
-      for (int b = 0; b < BP; ++b) h.v[b] = tp[b];
+      for (int b = 0; b < BP; ++b) th.h.v[b] = tp[b];
     }
   }
-  const double2 pf = post[theta * S + i];
-  for (int e = tid; e < S; e += C::T) tw[e] = twS[e];
+  th.pf = post[theta * S + th.i];
+  for (int e = th.tid; e < S; e += C::T) tw[e] = twS[e];
 
   const long ntiles = (nJ + C::TJ - 1) / C::TJ;
   const long t0 = (long)bg * tiles_per_block;
   const long t1 = min(ntiles, t0 + tiles_per_block);
   const long N = M * S;
 
-  for (long tile = t0; tile < t1; ++tile) {
-    const long J0 = tile * C::TJ;
-    // ---- stage the tile's input rows: contiguous in HBM (rows of S lanes), circular at M
-    {
-      const long g0 = J0 * DMU * S;
-#pragma unroll 4
-      for (int c = wave; c < C::XN / 64; c += NWAVES) {
-        long gi = g0 + c * 64 + lane;
-        if (gi >= N) gi -= N;
-        __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(X + c * 64), 16,
-                                         0, 0);
-      }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  if constexpr (C::DB) {
+    if (t0 < t1) dma_rows<C>(bufA, x, t0 * C::TJ * DMU * S, N, th.wave, th.lane);
+    wait_vmcnt<0>();
+    lds_barrier();
+    for (long tile = t0; tile < t1; ++tile) {
+      const bool odd = (tile - t0) & 1;
+      filter_tile<C, S, NMU, NT, DMU, BP, U, NP, CT>(th, odd ? bufB : bufA, odd ? bufA : bufB,
+                                                      tw, x, at, tile, tile + 1 < t1, N, Mh, nJ);
     }
-    __syncthreads();
-
-    // ---- FMA block
-    double2 acc[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) acc[u] = double2{0.0, 0.0};
-    conv_rows<0, (U - 1) * DMU + BP, S, U, DMU, BP, CT>(X + (w * U * DMU + p * BP) * S + i, h, acc);
-    __syncthreads();
-
-    // ---- tap-slice reduction (fixed order p = NP-1 .. 0) and post phase
-    const int gbase = (w * U) * NT + tl;
-    for (int q = NP - 1; q >= 0; --q) {
-      if (p == q) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          double2 a = acc[u];
-          double2 *gp = G + (gbase + u * NT) * C::GS + i;
-          if (q != NP - 1) a = cadd(a, *gp);
-          if (q == 0 && !CT) a = cmul(a, pf);
-          *gp = a;
-        }
-      }
-      if (q > 0) __syncthreads();
-    }
-    __syncthreads();
-
-    // ---- S-point DFT of each row
-    lds_fft<8>(G, S, C::GR, C::GS, 1, tw);
-
-    // ---- store segment-major: R = J*n_mu + theta
-#pragma unroll 2
-    for (int e = tid; e < C::GR * S; e += C::T) {
-      const int s = e / C::GR, r = e - s * C::GR;   // compile-time divisors
-      const long J = J0 + r / NT;
-      if (J < nJ) {
-        const long R = J * NMU + g * NT + r % NT;
-        at[(long)s * Mh + R] = G[r * C::GS + s];
-      }
-    }
-    __syncthreads();
+  } else {
+    lds_barrier();
+    for (long tile = t0; tile < t1; ++tile)
+      filter_tile<C, S, NMU, NT, DMU, BP, U, NP, CT>(th, bufA, bufA, tw, x, at, tile, false, N,
+                                                      Mh, nJ);
   }
 }
 
@@ -201,23 +333,17 @@ template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT
 static int launch(const SoiFilterArgs *a, hipStream_t st)
 {
   typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT> C;
+  static_assert(C::LDS_BYTES <= 160 * 1024, "filter tile exceeds LDS");
+  if ((long)C::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
   auto kern = k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT>;
-  const size_t lds = (size_t)(C::LDSN + S) * sizeof(double2);
-  static bool once = false;
-  if (!once) {
-    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    once = true;
-  }
   const long ntiles = (a->nJ + C::TJ - 1) / C::TJ;
   // blocks per phase group: enough to fill the CUs, tiles split into contiguous ranges
-  const int per_cu = (int)((160 * 1024) / lds) > 0 ? (int)((160 * 1024) / lds) : 1;
-  long nb = 256L * (per_cu > 2 ? 2 : per_cu) / C::NG;
+  long nb = 256L * C::BPC / C::NG;
   if (nb < 1) nb = 1;
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * C::NG)), dim3(C::T), lds, st,
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * C::NG)), dim3(C::T), 0, st,
                      (const double2 *)a->x, (double2 *)a->at, a->taps, a->post,
                      (const double2 *)a->twS, a->M, a->Mh, a->nJ, tpb);
   return (int)hipGetLastError();
@@ -230,16 +356,16 @@ extern "C" int soik_filter_shape(int S, int n_mu, int d_mu, int B, int ct, int *
   if (!ct) {
     if (n_mu != 5 || d_mu != 4) return -1;
     *NP = 2;
-    *U = 16;
+    *U = 8;
     *NT = 5;
     int half = (B + 1) / 2;
     *BP = half <= 32 ? 32 : half <= 36 ? 36 : half <= 40 ? 40 : 0;
     if (!*BP || B < 60) return -1;
-    switch (S) {
-      case 4: *W = 8; break;
-      case 8: *W = 4; break;
-      case 16: *W = 2; break;
-      case 32: *W = 1; break;
+    switch (S) {  /* 640 threads: S * 5 phases * 2 tap slices * W streams */
+      case 4: *W = 16; break;
+      case 8: *W = 8; break;
+      case 16: *W = 4; break;
+      case 32: *W = 2; break;
       case 64: *W = 1; break;
       default: return -1;
     }
@@ -248,15 +374,15 @@ extern "C" int soik_filter_shape(int S, int n_mu, int d_mu, int B, int ct, int *
   *NP = 4;
   int q = (B + 3) / 4;
   if (n_mu == 5 && d_mu == 4) {
-    *U = 16;
+    *U = 8;
     *BP = q <= 8 ? 8 : q <= 12 ? 12 : q <= 16 ? 16 : q <= 20 ? 20 : 0;
     if (!*BP) return -1;
-    switch (S) {
-      case 4: *NT = 5; *W = 4; break;
-      case 8: *NT = 5; *W = 2; break;
-      case 16: *NT = 5; *W = 1; break;
+    switch (S) {  /* S * NT * 4 tap slices * W threads */
+      case 4: *NT = 5; *W = 8; break;
+      case 8: *NT = 5; *W = 4; break;
+      case 16: *NT = 5; *W = 2; break;
       case 32: *NT = 5; *W = 1; break;
-      case 64: *NT = 1; *W = 1; break;
+      case 64: *NT = 1; *W = 2; break;
       default: return -1;
     }
     return 0;
@@ -284,30 +410,32 @@ extern "C" size_t soik_filter_lds_bytes(const SoiFilterArgs *a)
   const int rows = TJ * a->d_mu + a->NP * a->BP - a->d_mu;
   const long xn = ((long)rows * a->S + 63) / 64 * 64;
   const long gn = (long)TJ * a->NT * (a->S + 1);
-  return (size_t)((xn > gn ? xn : gn) + a->S) * sizeof(double2);
+  const long buf = xn > gn ? xn : gn;
+  const int db = 2 * buf * 16 + a->S * 16 <= 160 * 1024;
+  return (size_t)((db ? 2 : 1) * buf + a->S) * sizeof(double2);
 }
 
 #define RT_CASE(SV, BPV, WV)                                                        \
-  if (a->S == SV && a->BP == BPV) return launch<SV, 5, 5, 4, BPV, 16, 2, WV, false>(a, st);
+  if (a->S == SV && a->BP == BPV && a->W == WV) return launch<SV, 5, 5, 4, BPV, 8, 2, WV, false>(a, st);
 #define CT54_CASE(SV, NTV, BPV, WV)                                                 \
-  if (a->S == SV && a->BP == BPV) return launch<SV, 5, NTV, 4, BPV, 16, 4, WV, true>(a, st);
+  if (a->S == SV && a->BP == BPV) return launch<SV, 5, NTV, 4, BPV, 8, 4, WV, true>(a, st);
 #define CT87_CASE(SV, NTV, BPV, WV)                                                 \
   if (a->S == SV && a->BP == BPV) return launch<SV, 8, NTV, 7, BPV, 8, 4, WV, true>(a, st);
 
 extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
 {
   if (!a->ct) {
-    RT_CASE(4, 32, 8) RT_CASE(4, 36, 8) RT_CASE(4, 40, 8)
-    RT_CASE(8, 32, 4) RT_CASE(8, 36, 4) RT_CASE(8, 40, 4)
-    RT_CASE(16, 32, 2) RT_CASE(16, 36, 2) RT_CASE(16, 40, 2)
-    RT_CASE(32, 32, 1) RT_CASE(32, 36, 1) RT_CASE(32, 40, 1)
+    RT_CASE(4, 32, 16) RT_CASE(4, 36, 16) RT_CASE(4, 40, 16)
+    RT_CASE(8, 32, 8) RT_CASE(8, 36, 8) RT_CASE(8, 40, 8)
+    RT_CASE(16, 32, 4) RT_CASE(16, 36, 4) RT_CASE(16, 40, 4)
+    RT_CASE(32, 32, 2) RT_CASE(32, 36, 2) RT_CASE(32, 40, 2)
     RT_CASE(64, 32, 1) RT_CASE(64, 36, 1) RT_CASE(64, 40, 1)
   } else if (a->n_mu == 5) {
-    CT54_CASE(4, 5, 8, 4) CT54_CASE(4, 5, 12, 4) CT54_CASE(4, 5, 16, 4) CT54_CASE(4, 5, 20, 4)
-    CT54_CASE(8, 5, 8, 2) CT54_CASE(8, 5, 12, 2) CT54_CASE(8, 5, 16, 2) CT54_CASE(8, 5, 20, 2)
-    CT54_CASE(16, 5, 8, 1) CT54_CASE(16, 5, 12, 1) CT54_CASE(16, 5, 16, 1) CT54_CASE(16, 5, 20, 1)
+    CT54_CASE(4, 5, 8, 8) CT54_CASE(4, 5, 12, 8) CT54_CASE(4, 5, 16, 8) CT54_CASE(4, 5, 20, 8)
+    CT54_CASE(8, 5, 8, 4) CT54_CASE(8, 5, 12, 4) CT54_CASE(8, 5, 16, 4) CT54_CASE(8, 5, 20, 4)
+    CT54_CASE(16, 5, 8, 2) CT54_CASE(16, 5, 12, 2) CT54_CASE(16, 5, 16, 2) CT54_CASE(16, 5, 20, 2)
     CT54_CASE(32, 5, 8, 1) CT54_CASE(32, 5, 12, 1) CT54_CASE(32, 5, 16, 1) CT54_CASE(32, 5, 20, 1)
-    CT54_CASE(64, 1, 8, 1) CT54_CASE(64, 1, 12, 1) CT54_CASE(64, 1, 16, 1) CT54_CASE(64, 1, 20, 1)
+    CT54_CASE(64, 1, 8, 2) CT54_CASE(64, 1, 12, 2) CT54_CASE(64, 1, 16, 2) CT54_CASE(64, 1, 20, 2)
   } else {
     CT87_CASE(4, 8, 16, 2) CT87_CASE(4, 8, 20, 2)
     CT87_CASE(8, 8, 16, 1) CT87_CASE(8, 8, 20, 1)

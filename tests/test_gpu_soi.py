@@ -279,20 +279,32 @@ def test_stage_timing_over_calls(soi, orc, torch_cuda):
     assert rel(got, ref) < TOL
 
 
-@pytest.mark.parametrize("tc,tr", [(4, 4), (4, 8), (8, 4)])
-def test_fft_tile_width_variants(soi, torch_cuda, monkeypatch, tc, tr):
-    """The 4-wide tile variants of the register FFT passes (SOI_FFT_TC / SOI_FFT_TR) compute the
-    same transform as the default 8-wide tiles: M_hat = 1280 x 2048 at N = 2^24, P = 8."""
+FFT_VARIANTS = [
+    # (N, P, tuning environment): M_hat = 1280 x 2048 at N = 2^24, P = 8; 2560 x 2048 at P = 4
+    (2 ** 24, 8, {"SOI_FFT_TC": "4", "SOI_FFT_TR": "4"}),
+    (2 ** 24, 8, {"SOI_FFT_TC": "4"}),
+    (2 ** 24, 8, {"SOI_FFT_TR": "4"}),
+    (2 ** 24, 4, {"SOI_FFT_CVAR": "1"}),
+    (2 ** 24, 4, {"SOI_FFT_CVAR": "0"}),
+    (2 ** 24, 4, {"SOI_FFT_N1": "2048"}),
+    (2 ** 24, 4, {"SOI_FFT_N1": "2048", "SOI_FFT_TC": "4"}),
+]
+
+
+@pytest.mark.parametrize("N,P,env", FFT_VARIANTS)
+def test_fft_kernel_variants(soi, torch_cuda, monkeypatch, N, P, env):
+    """Every tuning alternative of the register FFT passes (tile width, LDS layout, register
+    budget, n1 x n2 split) computes the same transform as the default plan."""
+    print(f"case N={N} P={P} {env}", flush=True)
     torch = torch_cuda
-    N, P = 2 ** 24, 8
     xt = torch.empty(N, dtype=torch.complex128, device="cuda")
     assert soi.lib().soi_fill_uniform(xt.data_ptr(), N, 9, None) == 0
     ref = torch.fft.fft(xt)
     outs = []
-    for env in ({}, {"SOI_FFT_TC": str(tc), "SOI_FFT_TR": str(tr)}):
-        for kk in ("SOI_FFT_TC", "SOI_FFT_TR"):
+    for e in ({}, env):
+        for kk in ("SOI_FFT_TC", "SOI_FFT_TR", "SOI_FFT_CVAR", "SOI_FFT_N1"):
             monkeypatch.delenv(kk, raising=False)
-        for kk, v in env.items():
+        for kk, v in e.items():
             monkeypatch.setenv(kk, v)
         y = xt.clone()
         d = soi.make_descriptor(N, P)
@@ -303,5 +315,5 @@ def test_fft_tile_width_variants(soi, torch_cuda, monkeypatch, tc, tr):
             soi.free_soi_descriptor(d)
         outs.append(y)
     err = (torch.linalg.norm(outs[1] - outs[0]) / torch.linalg.norm(outs[0])).item()
-    assert err < 1e-15
+    assert err < (1e-15 if "SOI_FFT_N1" not in env else 3e-15)
     assert (torch.linalg.norm(outs[1] - ref) / torch.linalg.norm(ref)).item() < 3e-14

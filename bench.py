@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2n", type=int, default=25)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     return ap.parse_args()
 
 
@@ -77,6 +78,45 @@ def fft_split(Mh):
         if n2 in lens[:7] and n2 % 8 == 0 and (best is None or max(n1, n2) < max(best)):
             best = (n1, n2)
     return best if best else (Mh, 1)
+
+
+def bench_fp32(args, soi, torch, x64, N, P, fp64_gflops):
+    """The FP32 precision form on the same input rounded to complex64: throughput over the same
+    number of timed steps, and relative L2 error against rocFFT's FP64 FFT of the rounded input
+    (input rounding is not counted)."""
+    x = x64.to(torch.complex64)
+    d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    stream = torch.cuda.ExternalStream(soi.stream_of(d))
+    ref = torch.fft.fft(x.to(torch.complex128))
+    y = x.clone()
+    torch.cuda.synchronize()
+    soi.compute_soi(d, y.data_ptr())
+    err = (torch.linalg.norm(y.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+    del ref, y
+    torch.cuda.empty_cache()
+    for _ in range(args.warmup):
+        soi.compute_soi_async(d, x.data_ptr(), None)
+    torch.cuda.synchronize()
+    soi.enable_timing(d, True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        soi.compute_soi_async(d, x.data_ptr(), None)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    acc = soi.stage_times(d)
+    soi.enable_timing(d, False)
+    soi.free_soi_descriptor(d)
+    gflops = 5.0 * N * math.log2(N) / (ms * 1e-3) / 1e9
+    return {"value": round(gflops, 2), "unit": "GFLOP/s", "ms_per_step": round(ms, 4),
+            "dtype": "f32", "rel_l2_vs_rocfft_fp64": err,
+            "speedup_vs_fp64_form": round(gflops / fp64_gflops, 3),
+            "stages": {"filter_sdft_ms": round(acc[0], 4), "fft_pass1_ms": round(acc[2], 4),
+                       "fft_pass2_demod_ms": round(acc[3], 4)},
+            "form": "complex64 data; FP32 filter contraction (VALU) and FFTs; FP64 window, "
+                    "twiddle and W_inv generation; W_inv applied in FP64"}
 
 
 def cpu_baseline(log2n, P):
@@ -199,6 +239,8 @@ def main():
                              for k, v in kernels.items()},
         "roofline": roofline,
     }
+    if not args.no_fp32:
+        out["fp32_split"] = bench_fp32(args, soi, torch, x, N, P, gflops)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2n, P)
     print(json.dumps(out))

@@ -51,8 +51,11 @@ typedef double complex cfft_complex_t;
 
 /* Precision / form selector (not in the reference, which is FP64 only).
  *   SOI_PREC_FP64      plain FP64 path, the drop-in (default)
- *   SOI_PREC_FP32      precision form: complex64 input and output, FP32 filter contraction
- *                      and FFT arithmetic, FP64 windows, twiddles and demodulation */
+ *   SOI_PREC_FP32      precision form: complex64 input, output and alpha_tilde (a caller that
+ *                      preallocates alpha_tilde sizes it S*M_hat complex64 values); FP32 filter
+ *                      contraction and FFT arithmetic with windows and twiddles generated in
+ *                      FP64 and rounded once; W_inv applied in FP64.  Not combinable with
+ *                      use_vlc (the VLC words encode FP64 values). */
 enum { SOI_PREC_FP64 = 0, SOI_PREC_FP32 = 1 };
 
 /* Window evaluation (not in the reference).
@@ -118,7 +121,8 @@ void set_default_soi_descriptor(soi_desc_t *desc);
 void init_soi_descriptor(soi_desc_t *desc, MPI_Comm comm, cfft_size_t k);
 
 /* pfft.c:276-721: in-place forward DFT of alpha_dt (N elements, device or host pointer).
- * With precision == SOI_PREC_FP32 the buffer holds N complex64 values instead. */
+ * With precision == SOI_PREC_FP32 the buffer holds N complex64 values instead (passed through
+ * the same cfft_complex_t * parameter). */
 void compute_soi(soi_desc_t *d, cfft_complex_t *alpha_dt);
 
 /* pfft.c:234-267 */

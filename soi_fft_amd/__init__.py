@@ -215,11 +215,14 @@ def make_descriptor(N, P, k=1, n_mu=5, d_mu=4, B=72, tau=None, sigma=None, use_v
 
 
 def soi_fft(x, P, k=1, **params):
-    """Convenience: in-place SOI FFT of a CUDA(HIP) complex128 torch tensor of length N.
+    """Convenience: in-place SOI FFT of a CUDA(HIP) complex128 (FP64 form) or complex64 (FP32
+    form) torch tensor of length N.
 
     Returns x (overwritten with the N-point forward DFT, natural order)."""
     import torch  # plumbing only: device memory and synchronisation
-    assert x.is_cuda and x.dtype == torch.complex128 and x.is_contiguous()
+    assert x.is_cuda and x.is_contiguous() and x.dtype in (torch.complex128, torch.complex64)
+    if x.dtype == torch.complex64:
+        params.setdefault("precision", SOI_PREC_FP32)
     d = make_descriptor(x.numel(), P, k, **params)
     try:
         torch.cuda.synchronize()

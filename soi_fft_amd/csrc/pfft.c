@@ -81,6 +81,9 @@ typedef struct {
   int ref_windows;
   void *taps;
   soi_c64_t *post, *twS;
+  /* FP32 form: copies of the FP64 tables rounded to FP32 (W_inv stays FP64) */
+  int f32;
+  soi_c32_t *taps32, *post32, *twS32, *tw1_32, *tw2_32, *thi32, *tlo32;
   /* fft */
   int fft_single, single_count, fft_reg;
   int n1, n2, Tc, Tr, B1, lb1, clay;
@@ -289,6 +292,13 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->taps);
   (void)hipFree(pl->post);
   (void)hipFree(pl->twS);
+  (void)hipFree(pl->taps32);
+  (void)hipFree(pl->post32);
+  (void)hipFree(pl->twS32);
+  (void)hipFree(pl->tw1_32);
+  (void)hipFree(pl->tw2_32);
+  (void)hipFree(pl->thi32);
+  (void)hipFree(pl->tlo32);
   (void)hipFree(pl->tw1);
   (void)hipFree(pl->tw2);
   (void)hipFree(pl->thi);
@@ -331,8 +341,13 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
             d->n_mu, d->d_mu);
     return;
   }
-  if (d->precision != SOI_PREC_FP64) {
-    set_err(SOI_ERR_ARGS, "precision %d is not available in this build", d->precision);
+  if (d->precision != SOI_PREC_FP64 && d->precision != SOI_PREC_FP32) {
+    set_err(SOI_ERR_ARGS, "unknown precision %d", d->precision);
+    return;
+  }
+  if (d->precision == SOI_PREC_FP32 && d->use_vlc) {
+    set_err(SOI_ERR_ARGS, "use_vlc needs precision SOI_PREC_FP64: the VLC words encode FP64 "
+            "values (compress.c)");
     return;
   }
   long S = (long)k * d->P;
@@ -375,6 +390,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
   pl->B = (int)d->B;
   pl->Mh = (long)d->n_mu * M / d->d_mu;
   pl->l = pl->Mh / d->P;
+  pl->f32 = d->precision == SOI_PREC_FP32;
   if (plan_filter(pl, d->window_mode) || plan_fft(pl)) goto fail_quiet;
   HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
 
@@ -391,12 +407,13 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     HIPCHK(hipMalloc((void **)&d->W_inv, (size_t)M * sizeof(soi_c64_t)));
     HIPCHK(hipMalloc((void **)&pl->twS, (size_t)S * sizeof(soi_c64_t)));
     d->w_dup = pl->taps;
+    const size_t esz = pl->f32 ? sizeof(soi_c32_t) : sizeof(soi_c64_t);
     if (!d->alpha_tilde) {
-      hipError_t e = hipMalloc((void **)&d->alpha_tilde, (size_t)S * pl->Mh * sizeof(soi_c64_t));
+      hipError_t e = hipMalloc((void **)&d->alpha_tilde, (size_t)S * pl->Mh * esz);
       if (e != hipSuccess) {
         d->alpha_tilde = NULL;
         set_err(SOI_ERR_ALLOC, "Failed to allocate d->alpha_tilde (%zu bytes): %s",
-                (size_t)S * pl->Mh * sizeof(soi_c64_t), hipGetErrorString(e));
+                (size_t)S * pl->Mh * esz, hipGetErrorString(e));
         goto fail_quiet;
       }
       pl->owns_at = 1;
@@ -421,6 +438,34 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       KCHK(soik_make_twiddles(pl->tw2, pl->n2, pl->n2, 1, pl->stream));
       KCHK(soik_make_twiddles(pl->thi, nhi, pl->Mh, pl->B1, pl->stream));
       KCHK(soik_make_twiddles(pl->tlo, pl->B1, pl->Mh, 1, pl->stream));
+    }
+    if (pl->f32) {
+      /* FP32 copies of the tables, rounded from the FP64 values generated above */
+      size_t nt = ntap * (pl->fa.ct ? 2 : 1); /* doubles; even, since S is a multiple of 4 */
+      HIPCHK(hipMalloc((void **)&pl->taps32, nt * sizeof(float)));
+      KCHK(soik_f64_to_f32((const soi_c64_t *)pl->taps, pl->taps32, (long)(nt / 2), pl->stream));
+      HIPCHK(hipMalloc((void **)&pl->post32, (size_t)d->n_mu * S * sizeof(soi_c32_t)));
+      KCHK(soik_f64_to_f32(pl->post, pl->post32, (long)d->n_mu * S, pl->stream));
+      HIPCHK(hipMalloc((void **)&pl->twS32, (size_t)S * sizeof(soi_c32_t)));
+      KCHK(soik_f64_to_f32(pl->twS, pl->twS32, S, pl->stream));
+      pl->fa.taps = pl->taps32;
+      pl->fa.post = pl->post32;
+      pl->fa.twS = pl->twS32;
+      pl->fa.f32 = 1;
+      if (pl->fft_single) {
+        HIPCHK(hipMalloc((void **)&pl->tw1_32, (size_t)pl->Mh * sizeof(soi_c32_t)));
+        KCHK(soik_f64_to_f32(pl->tw1, pl->tw1_32, pl->Mh, pl->stream));
+      } else {
+        long nhi = (pl->Mh + pl->B1 - 1) / pl->B1;
+        HIPCHK(hipMalloc((void **)&pl->tw1_32, (size_t)pl->n1 * sizeof(soi_c32_t)));
+        HIPCHK(hipMalloc((void **)&pl->tw2_32, (size_t)pl->n2 * sizeof(soi_c32_t)));
+        HIPCHK(hipMalloc((void **)&pl->thi32, (size_t)nhi * sizeof(soi_c32_t)));
+        HIPCHK(hipMalloc((void **)&pl->tlo32, (size_t)pl->B1 * sizeof(soi_c32_t)));
+        KCHK(soik_f64_to_f32(pl->tw1, pl->tw1_32, pl->n1, pl->stream));
+        KCHK(soik_f64_to_f32(pl->tw2, pl->tw2_32, pl->n2, pl->stream));
+        KCHK(soik_f64_to_f32(pl->thi, pl->thi32, nhi, pl->stream));
+        KCHK(soik_f64_to_f32(pl->tlo, pl->tlo32, pl->B1, pl->stream));
+      }
     }
     if (d->use_vlc) {
       long nch = S * d->P;
@@ -534,7 +579,7 @@ static void vlc_plan(soi_desc_t *d, soi_plan_t *pl)
   free(pcs);
 }
 
-static int run_filter(soi_desc_t *d, soi_plan_t *pl, const soi_c64_t *x, hipStream_t st)
+static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, hipStream_t st)
 {
   pl->fa.x = x;
   pl->fa.at = d->alpha_tilde;
@@ -579,8 +624,37 @@ fail:
   return -1;
 }
 
-static int run_fft(soi_desc_t *d, soi_plan_t *pl, soi_c64_t *X, hipStream_t st)
+static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, hipStream_t st)
 {
+  soi_c32_t *at = (soi_c32_t *)d->alpha_tilde;
+  const soi_c64_t *winv = (const soi_c64_t *)d->W_inv;
+  if (pl->fft_single) {
+    mark(pl, 2, st);
+    KCHK(soik_fft_single_f32(at, X, pl->tw1_32, winv, pl->Mh, pl->M, (int)pl->S,
+                             pl->single_count, st));
+    mark(pl, 3, st);
+  } else if (pl->fft_reg) {
+    KCHK(soik_fft_cols_reg_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->lb1,
+                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, st));
+    mark(pl, 3, st);
+    KCHK(soik_fft_rows_reg_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
+                               (int)pl->S, pl->Tr, st));
+  } else {
+    KCHK(soik_fft_col_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->Tc, pl->B1,
+                          pl->Mh, (int)pl->S, st));
+    mark(pl, 3, st);
+    KCHK(soik_fft_row_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Tr, pl->Mh, pl->M,
+                          (int)pl->S, st));
+  }
+  return 0;
+fail:
+  return -1;
+}
+
+static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, hipStream_t st)
+{
+  if (pl->f32) return run_fft_f32(d, pl, (soi_c32_t *)Xv, st);
+  soi_c64_t *X = (soi_c64_t *)Xv;
   soi_c64_t *at = (soi_c64_t *)d->alpha_tilde;
   if (pl->fft_single) {
     mark(pl, 2, st);
@@ -623,7 +697,7 @@ static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, 
   {
     int dev = is_device_ptr(alpha_dt);
     soi_c64_t *buf = (soi_c64_t *)alpha_dt;
-    size_t bytes = (size_t)pl->N * sizeof(soi_c64_t);
+    size_t bytes = (size_t)pl->N * (pl->f32 ? sizeof(soi_c32_t) : sizeof(soi_c64_t));
     if (!dev) {
       if (!pl->io) {
         hipError_t e = hipMalloc((void **)&pl->io, bytes);

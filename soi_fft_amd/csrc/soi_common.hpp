@@ -13,12 +13,23 @@ namespace soi {
 template <typename T> struct cplx;
 template <> struct cplx<double> { typedef double2 t; };
 template <> struct cplx<float> { typedef float2 t; };
+template <typename C> struct real_of;
+template <> struct real_of<double2> { typedef double t; };
+template <> struct real_of<float2> { typedef float t; };
+template <typename C> using real_t = typename real_of<C>::t;
 
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
   return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// W_inv[m] * y in FP64, rounded to the precision of y (the demodulation gain amplifies error
+// near segment edges, so the product is formed in FP64 in both forms)
+template <typename C> __device__ __forceinline__ C demod(double2 w, C y)
+{
+  const double yx = (double)y.x, yy = (double)y.y;
+  return C{(real_t<C>)(w.x * yx - w.y * yy), (real_t<C>)(w.x * yy + w.y * yx)};
 }
 // multiply by -i
 template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return C{a.y, -a.x}; }

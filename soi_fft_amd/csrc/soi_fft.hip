@@ -127,8 +127,8 @@ template <int L, int TG> struct SeqMajor {
 // stage s (radix R, span Ns) for butterfly j land at (j - j%Ns)*R + j%Ns + k*Ns; stage s+1
 // (radix R2) butterfly j2 reads j2 + k*L/R2.  Done in parts: real then imaginary halves, and
 // NQP groups of the T sequences, so the LDS footprint is L*(T/NQP)*8 bytes (plus padding).
-template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2, typename LY>
-__device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, int ts)
+template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2, typename LY, typename C>
+__device__ __forceinline__ void exchange(C (&v)[E], real_t<C> *lds, int q, int ts)
 {
   constexpr int NB = E / R, NB2 = E / R2, TG = T / NQP;
   const int qg = q / TG, ql = q % TG;
@@ -154,7 +154,7 @@ __device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, in
           const int j = ts + TS * c;
 #pragma unroll
           for (int k = 0; k < R2; ++k) {
-            const double val = lds[LY::at(j + k * (L / R2), ql)];
+            const real_t<C> val = lds[LY::at(j + k * (L / R2), ql)];
             if (half) v[c * R2 + k].y = val; else v[c * R2 + k].x = val;
           }
         }
@@ -165,8 +165,8 @@ __device__ __forceinline__ void exchange(double2 (&v)[E], double *lds, int q, in
 }
 
 // twiddle + DFT of every butterfly of stage s held in registers
-template <int L, int E, int TS, int R, int NS>
-__device__ __forceinline__ void stage(double2 (&v)[E], int ts, const double2 *__restrict__ tw)
+template <int L, int E, int TS, int R, int NS, typename C>
+__device__ __forceinline__ void stage(C (&v)[E], int ts, const C *__restrict__ tw)
 {
   constexpr int NB = E / R;
 #pragma unroll
@@ -183,16 +183,16 @@ __device__ __forceinline__ void stage(double2 (&v)[E], int ts, const double2 *__
 }
 
 // Run stages 0..n-1 on register data already loaded for stage 0.
-template <int L, int E, int T, int NQP, int TS, typename RS, typename LY, int S0>
-__device__ __forceinline__ void run_stages(double2 (&v)[E], double *lds, int q, int ts,
-                                           const double2 *__restrict__ tw)
+template <int L, int E, int T, int NQP, int TS, typename RS, typename LY, int S0, typename C>
+__device__ __forceinline__ void run_stages(C (&v)[E], real_t<C> *lds, int q, int ts,
+                                           const C *__restrict__ tw)
 {
   constexpr int R = RS::at(S0);
   constexpr int NS = RS::span(S0);
-  stage<L, E, TS, R, NS>(v, ts, tw);
+  stage<L, E, TS, R, NS, C>(v, ts, tw);
   if constexpr (S0 + 1 < RS::n) {
-    exchange<L, E, T, NQP, TS, R, NS, RS::at(S0 + 1), LY>(v, lds, q, ts);
-    run_stages<L, E, T, NQP, TS, RS, LY, S0 + 1>(v, lds, q, ts, tw);
+    exchange<L, E, T, NQP, TS, R, NS, RS::at(S0 + 1), LY, C>(v, lds, q, ts);
+    run_stages<L, E, T, NQP, TS, RS, LY, S0 + 1, C>(v, lds, q, ts, tw);
   }
 }
 
@@ -214,15 +214,16 @@ __device__ __forceinline__ int out_index(int ts, int c, int k)
 // waves per SIMD the register budget is sized for, with BPC blocks resident per CU
 constexpr int fft_minw(int BPC, int threads) { return (BPC * threads / 64 + 3) / 4; }
 
-template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY>
-__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_cols(double2 *__restrict__ data,
-                                                          const double2 *__restrict__ tw,
-                                                          const double2 *__restrict__ thi,
-                                                          const double2 *__restrict__ tlo,
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C>
+__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_cols(C *__restrict__ data,
+                                                          const C *__restrict__ tw,
+                                                          const C *__restrict__ thi,
+                                                          const C *__restrict__ tlo,
                                                           int n2, int lb1, long Mh, int S)
 {
   constexpr int TS = L / E;
-  extern __shared__ double ldsd[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
   const int tid = threadIdx.x;
   const int q = tid % T, ts = tid / T;
   const int ncb = n2 / T;
@@ -230,17 +231,17 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
   const int cb = bid % ncb, s = bid / ncb;
   if (s >= S) return;
   const int c0 = cb * T;
-  double2 *seg = data + (long)s * Mh + c0 + q;
+  C *seg = data + (long)s * Mh + c0 + q;
   const int n2s = n2;
 
-  double2 v[E];
+  C v[E];
   constexpr int R0 = RS::at(0);
 #pragma unroll
   for (int c = 0; c < E / R0; ++c)
 #pragma unroll
     for (int k = 0; k < R0; ++k) v[c * R0 + k] = seg[(unsigned)(ts + TS * c + k * (L / R0)) * (unsigned)n2s];
 
-  run_stages<L, E, T, NQP, TS, RS, LY, 0>(v, ldsd, q, ts, tw);
+  run_stages<L, E, T, NQP, TS, RS, LY, 0, C>(v, ldsd, q, ts, tw);
 
   constexpr int RL = RS::at(RS::n - 1);
 #pragma unroll
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
     for (int k = 0; k < RL; ++k) {
       const int m1 = out_index<L, TS, RS>(ts, c, k);
       const unsigned ex = (unsigned)(c0 + q) * (unsigned)m1;   // < M_hat < 2^32
-      const double2 t = cmul(thi[ex >> lb1], tlo[ex & ((1u << lb1) - 1)]);
+      const C t = cmul(thi[ex >> lb1], tlo[ex & ((1u << lb1) - 1)]);
       seg[(unsigned)m1 * (unsigned)n2s] = cmul(v[c * RL + k], t);
     }
 }
@@ -257,15 +258,16 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 // ---------------------------------------------------------------------------------------------
 // Pass 2: T adjacent rows of length L = n2, demodulated, stored transposed with 128 B runs
 // ---------------------------------------------------------------------------------------------
-template <int L, int E, int T, int NQP, int BPC, typename RS>
-__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_rows(const double2 *__restrict__ data,
-                                                          double2 *__restrict__ X,
-                                                          const double2 *__restrict__ tw,
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename C>
+__global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_rows(const C *__restrict__ data,
+                                                          C *__restrict__ X,
+                                                          const C *__restrict__ tw,
                                                           const double2 *__restrict__ winv,
                                                           int n1, long Mh, long M, int S)
 {
   constexpr int TS = L / E;
-  extern __shared__ double ldsd[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
   const int tid = threadIdx.x;
   // loads: consecutive threads walk a row (coalesced); the sequence index is the slow one
   const int ts = tid % TS, q = tid / TS;
@@ -275,9 +277,9 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
   const int rb = bid / S;
   if (rb >= nrb) return;
   const int a = rb * T;
-  const double2 *row = data + (long)s * Mh + (long)(a + q) * L;
+  const C *row = data + (long)s * Mh + (long)(a + q) * L;
 
-  double2 v[E];
+  C v[E];
   constexpr int R0 = RS::at(0);
 #pragma unroll
   for (int c = 0; c < E / R0; ++c)
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
     for (int k = 0; k < R0; ++k) v[c * R0 + k] = row[ts + TS * c + k * (L / R0)];
 
   using LY = SeqMajor<L, T / NQP>;
-  run_stages<L, E, T, NQP, TS, RS, LY, 0>(v, ldsd, q, ts, tw);
+  run_stages<L, E, T, NQP, TS, RS, LY, 0, C>(v, ldsd, q, ts, tw);
 
   // transpose through LDS so that stores run over the T rows (m = a + q + n1*m2), in NQP groups
   // of T/NQP rows and real/imaginary halves
@@ -293,11 +295,11 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
   constexpr int TG = T / NQP;
   constexpr int NT = T * TS;
   constexpr int EG = E / NQP;  // values each thread stores per group
-  double2 *out = X + (long)s * M;
+  C *out = X + (long)s * M;
   const int qg = q / TG, ql = q % TG;
 #pragma unroll
   for (int g = 0; g < NQP; ++g) {
-    double2 w[EG];
+    C w[EG];
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
       if (qg == g) {
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 #pragma unroll
       for (int c = 0; c < EG; ++c) {
         const int e = tid + c * NT;  // e = m2*TG + qq
-        const double val = ldsd[LY::at(e / TG, e % TG)];
+        const real_t<C> val = ldsd[LY::at(e / TG, e % TG)];
         if (half) w[c].y = val; else w[c].x = val;
       }
       __syncthreads();
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
     for (int c = 0; c < EG; ++c) {
       const int e = tid + c * NT;
       const long m = (long)(a + g * TG + e % TG) + (long)n1 * (e / TG);
-      if (m < M) out[m] = cmul(winv[m], w[c]);
+      if (m < M) out[m] = demod<C>(winv[m], w[c]);  // W_inv applied in FP64
     }
   }
 }
@@ -328,12 +330,12 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 // ---------------------------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------------------------
-template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY>
-static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, const double2 *tlo,
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C>
+static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
                        int n2, int lb1, long Mh, int S, hipStream_t st)
 {
-  auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY>;
-  constexpr size_t lds = (size_t)LY::size * 8;
+  auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C>;
+  constexpr size_t lds = (size_t)LY::size * sizeof(real_t<C>);
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -345,12 +347,12 @@ static int launch_cols(double2 *data, const double2 *tw, const double2 *thi, con
   return (int)hipGetLastError();
 }
 
-template <int L, int E, int T, int NQP, int BPC, typename RS>
-static int launch_rows(const double2 *data, double2 *X, const double2 *tw, const double2 *winv,
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename C>
+static int launch_rows(const C *data, C *X, const C *tw, const double2 *winv,
                        int n1, long Mh, long M, int S, hipStream_t st)
 {
-  auto kern = k_fft_rows<L, E, T, NQP, BPC, RS>;
-  constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * 8;
+  auto kern = k_fft_rows<L, E, T, NQP, BPC, RS, C>;
+  constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * sizeof(real_t<C>);
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -384,14 +386,14 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 // Tile width T: 8 (128-byte runs) or 4 (64-byte runs).  variant selects a tuning alternative
 // for some lengths: 1 = one padded LDS run per sequence (2048, 2560 at T = 8); 2 = the whole
 // 256-VGPR budget for one 4-wide block per CU (2560 at T = 4).
-extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
-                                 const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                 int T, int variant, hipStream_t st)
+template <typename C>
+static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int lb1,
+                    long Mh, int S, int T, int variant, hipStream_t st)
 {
   if (n2 % T) return (int)hipErrorInvalidValue;
 #define COLS(VV, LV, EV, TV, NQ, BPC, LY, ...) \
   if (n1 == LV && T == TV && variant == VV) \
-    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>, C>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
   COLS(0, 256, 16, 8, 1, 1, QFast, 16, 16) COLS(0, 320, 20, 8, 1, 1, QFast, 10, 4, 4, 2)
   COLS(0, 512, 16, 8, 1, 1, QFast, 16, 16, 2) COLS(0, 640, 20, 8, 1, 1, QFast, 10, 4, 4, 4)
   COLS(0, 1024, 16, 8, 1, 1, QFast, 16, 16, 4) COLS(0, 1280, 20, 8, 1, 1, QFast, 10, 4, 4, 4, 2)
@@ -404,13 +406,13 @@ extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double
   return (int)hipErrorInvalidValue;
 }
 
-extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 *tw2,
-                                 const double2 *winv, int n1, int n2, long Mh, long M, int S,
-                                 int T, hipStream_t st)
+template <typename C>
+static int rows_reg(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2,
+                    long Mh, long M, int S, int T, hipStream_t st)
 {
   if (n1 % T) return (int)hipErrorInvalidValue;
 #define ROWS(LV, EV, TV, NQ, BPC, ...) \
-  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>>(data, X, tw2, winv, n1, Mh, M, S, st);
+  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, st);
   ROWS(256, 16, 8, 1, 1, 16, 16) ROWS(320, 20, 8, 1, 1, 10, 4, 4, 2) ROWS(512, 16, 8, 1, 1, 16, 16, 2)
   ROWS(640, 20, 8, 1, 1, 10, 4, 4, 4) ROWS(1024, 16, 8, 1, 1, 16, 16, 4)
   ROWS(1280, 20, 8, 1, 1, 10, 4, 4, 4, 2) ROWS(2048, 16, 8, 1, 1, 16, 16, 8)
@@ -419,4 +421,33 @@ extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 
   ROWS(2048, 16, 4, 1, 2, 16, 16, 8)
 #undef ROWS
   return (int)hipErrorInvalidValue;
+}
+
+extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
+                                 const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
+                                 int T, int variant, hipStream_t st)
+{
+  return cols_reg<double2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, st);
+}
+
+extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 *tw2,
+                                 const double2 *winv, int n1, int n2, long Mh, long M, int S,
+                                 int T, hipStream_t st)
+{
+  return rows_reg<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, st);
+}
+
+// FP32 form: complex64 data and tables (rounded from their FP64 values); W_inv stays FP64.
+extern "C" int soik_fft_cols_reg_f32(float2 *data, const float2 *tw1, const float2 *thi,
+                                     const float2 *tlo, int n1, int n2, int lb1, long Mh, int S,
+                                     int T, int variant, hipStream_t st)
+{
+  return cols_reg<float2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, st);
+}
+
+extern "C" int soik_fft_rows_reg_f32(const float2 *data, float2 *X, const float2 *tw2,
+                                     const double2 *winv, int n1, int n2, long Mh, long M, int S,
+                                     int T, hipStream_t st)
+{
+  return rows_reg<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, st);
 }

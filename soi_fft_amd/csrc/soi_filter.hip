@@ -29,14 +29,14 @@ using namespace soi;
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 
-template <int BP, bool CT> struct Taps;
-template <int BP> struct Taps<BP, false> { double v[BP]; };
-template <int BP> struct Taps<BP, true> { double re[BP], im[BP]; };
+template <int BP, bool CT, typename RT> struct Taps;
+template <int BP, typename RT> struct Taps<BP, false, RT> { RT v[BP]; };
+template <int BP, typename RT> struct Taps<BP, true, RT> { RT re[BP], im[BP]; };
 
 // Fully unrolled accumulation: input row o of the thread's column feeds accumulator u with tap
 // b = o - u*d whenever 0 <= b < BP (constant indices: taps and accumulators stay in registers).
-template <int O, int UU, int U, int DMU, int BP, bool CT>
-__device__ __forceinline__ void conv_taps(const double2 xv, const Taps<BP, CT> &h, double2 (&acc)[U])
+template <int O, int UU, int U, int DMU, int BP, bool CT, typename C>
+__device__ __forceinline__ void conv_taps(const C xv, const Taps<BP, CT, real_t<C>> &h, C (&acc)[U])
 {
   if constexpr (UU < U) {
     constexpr int b = O - UU * DMU;
@@ -51,37 +51,38 @@ __device__ __forceinline__ void conv_taps(const double2 xv, const Taps<BP, CT> &
         acc[UU].y = fma(h.v[b], xv.y, acc[UU].y);
       }
     }
-    conv_taps<O, UU + 1, U, DMU, BP, CT>(xv, h, acc);
+    conv_taps<O, UU + 1, U, DMU, BP, CT, C>(xv, h, acc);
   }
 }
 
-template <int O, int OEND, int S, int U, int DMU, int BP, bool CT>
-__device__ __forceinline__ void conv_rows(const double2 *__restrict__ col, const Taps<BP, CT> &h,
-                                          double2 (&acc)[U])
+template <int O, int OEND, int S, int U, int DMU, int BP, bool CT, typename C>
+__device__ __forceinline__ void conv_rows(const C *__restrict__ col, const Taps<BP, CT, real_t<C>> &h,
+                                          C (&acc)[U])
 {
   if constexpr (O < OEND) {
-    const double2 xv = col[O * S];
-    conv_taps<O, 0, U, DMU, BP, CT>(xv, h, acc);
-    conv_rows<O + 1, OEND, S, U, DMU, BP, CT>(col, h, acc);
+    const C xv = col[O * S];
+    conv_taps<O, 0, U, DMU, BP, CT, C>(xv, h, acc);
+    conv_rows<O + 1, OEND, S, U, DMU, BP, CT, C>(col, h, acc);
   }
 }
 
-template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
+template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C>
 struct FilterCfg {
+  static constexpr int EPL = 16 / (int)sizeof(C);    // values per 16-byte direct-to-LDS load
   static constexpr int T = S * NT * NP * W;          // threads
   static constexpr int NWAVES = T / 64;
   static constexpr int TJ = U * W;                   // row groups per tile
   static constexpr int ROWS = TJ * DMU + NP * BP - DMU;  // input rows per tile
-  static constexpr int XN = (ROWS * S + 63) / 64 * 64;   // staged complex values (whole waves)
-  static constexpr int NDMA = XN / 64;               // wave-wide 16-byte loads per tile
+  static constexpr int XN = (ROWS * S + 64 * EPL - 1) / (64 * EPL) * (64 * EPL);  // staged values
+  static constexpr int NDMA = XN / (64 * EPL);       // wave-wide 16-byte loads per tile
   static constexpr int GR = TJ * NT;                 // output rows per tile (this block's phases)
   static constexpr int GS = S + 1;                   // padded G row stride
   static constexpr int GN = GR * GS;
   static constexpr int BUFN = (XN > GN ? XN : GN);   // one tile buffer (G reuses it)
   static constexpr int NG = NMU / NT;                // phase groups
   // two tile buffers (the next tile's rows stream in while this one computes) when they fit
-  static constexpr bool DB = 2 * BUFN * 16 + S * 16 <= 160 * 1024;
-  static constexpr int LDS_BYTES = ((DB ? 2 : 1) * BUFN + S) * 16;
+  static constexpr bool DB = (2 * BUFN + S) * (int)sizeof(C) <= 160 * 1024;
+  static constexpr int LDS_BYTES = ((DB ? 2 : 1) * BUFN + S) * (int)sizeof(C);
   // blocks resident per CU (LDS and registers), and the register budget that follows
   static constexpr int BPC = (LDS_BYTES <= 80 * 1024 && T <= 320) ? 2 : 1;
   static constexpr int MINW = (BPC * NWAVES + 3) / 4;
@@ -122,21 +123,21 @@ template <int S> constexpr int sdft_radix(int pass)
   return 0;
 }
 
-template <int S, int GR, int GS, int T, int P, int NS>
-__device__ __forceinline__ void sdft_passes(double2 *G, const double2 *tw, int tid)
+template <int S, int GR, int GS, int T, int P, int NS, typename C>
+__device__ __forceinline__ void sdft_passes(C *G, const C *tw, int tid)
 {
   constexpr int R = sdft_radix<S>(P);
   if constexpr (R != 0) {
     constexpr int NBF = S / R;                 // butterflies per row
     constexpr int TOT = GR * NBF;
     constexpr int NB = (TOT + T - 1) / T;
-    double2 v[NB][R];
+    C v[NB][R];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       const int g = tid + c * T;
       if (TOT % T == 0 || g < TOT) {
         const int q = g / NBF, j = g % NBF;
-        const double2 *src = G + q * GS + j;
+        const C *src = G + q * GS + j;
 #pragma unroll
         for (int k = 0; k < R; ++k) v[c][k] = src[k * NBF];
       }
@@ -154,35 +155,36 @@ __device__ __forceinline__ void sdft_passes(double2 *G, const double2 *tw, int t
           for (int k = 1; k < R; ++k) v[c][k] = cmul(v[c][k], tw[k * jm * step]);
         }
         dft_r<R>(v[c]);
-        double2 *dst = G + q * GS + (j - jm) * R + jm;
+        C *dst = G + q * GS + (j - jm) * R + jm;
 #pragma unroll
         for (int k = 0; k < R; ++k) dst[k * NS] = v[c][k];
       }
     }
     lds_barrier();
-    sdft_passes<S, GR, GS, T, P + 1, NS * R>(G, tw, tid);
+    sdft_passes<S, GR, GS, T, P + 1, NS * R, C>(G, tw, tid);
   }
 }
 
 // Stage the XN input values of a tile (contiguous from element g0, wrapping once at N) into
 // dst with direct-to-LDS loads; wave w moves 64-value chunks w, w + NWAVES, ...
-template <typename C>
-__device__ __forceinline__ void dma_rows(double2 *dst, const double2 *__restrict__ x, long g0,
-                                         long N, int wave, int lane)
+template <typename CF, typename C>
+__device__ __forceinline__ void dma_rows(C *dst, const C *__restrict__ x, long g0, long N,
+                                         int wave, int lane)
 {
+  constexpr int CH = 64 * CF::EPL;  // values per wave-wide load
 #pragma unroll
-  for (int c = wave; c < C::NDMA; c += C::NWAVES) {
-    long gi = g0 + c * 64 + lane;
+  for (int c = wave; c < CF::NDMA; c += CF::NWAVES) {
+    long gi = g0 + c * CH + lane * CF::EPL;  // N and g0 are multiples of EPL: no pair straddles N
     if (gi >= N) gi -= N;
-    __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * 64), 16,
+    __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * CH), 16,
                                      0, 0);
   }
 }
 
 // Per-thread state that stays in registers for the whole kernel.
-template <int BP, bool CT> struct FilterThread {
-  Taps<BP, CT> h;
-  double2 pf;        // post phase
+template <int BP, bool CT, typename C> struct FilterThread {
+  Taps<BP, CT, real_t<C>> h;
+  C pf;              // post phase
   int i, tl, p, w, g;
   int tid, wave, lane;
 };
@@ -190,40 +192,40 @@ template <int BP, bool CT> struct FilterThread {
 // One tile: prefetch the next tile's rows into nxt (when there is a next tile), FMA block from
 // cur, tap-slice reduction + post phase + S-point DFT in cur, stores.  Barriers wait on LDS
 // traffic only: the prefetch and the stores stay in flight.
-template <typename C, int S, int NMU, int NT, int DMU, int BP, int U, int NP, bool CT>
-__device__ __forceinline__ void filter_tile(const FilterThread<BP, CT> &th, double2 *cur,
-                                            double2 *nxt, const double2 *tw,
-                                            const double2 *__restrict__ x,
-                                            double2 *__restrict__ at, long tile, bool more,
-                                            long N, long Mh, long nJ)
+template <typename CF, int S, int NMU, int NT, int DMU, int BP, int U, int NP, bool CT,
+          typename C>
+__device__ __forceinline__ void filter_tile(const FilterThread<BP, CT, C> &th, C *cur, C *nxt,
+                                            const C *tw, const C *__restrict__ x,
+                                            C *__restrict__ at, long tile, bool more, long N,
+                                            long Mh, long nJ)
 {
-  const long J0 = tile * C::TJ;
-  if constexpr (C::DB) {
-    if (more) dma_rows<C>(nxt, x, (tile + 1) * C::TJ * DMU * S, N, th.wave, th.lane);
+  const long J0 = tile * CF::TJ;
+  if constexpr (CF::DB) {
+    if (more) dma_rows<CF, C>(nxt, x, (tile + 1) * CF::TJ * DMU * S, N, th.wave, th.lane);
   } else {
-    dma_rows<C>(cur, x, tile * C::TJ * DMU * S, N, th.wave, th.lane);
+    dma_rows<CF, C>(cur, x, tile * CF::TJ * DMU * S, N, th.wave, th.lane);
     wait_vmcnt<0>();
     lds_barrier();
   }
 
   // ---- FMA block
-  double2 acc[U];
+  C acc[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) acc[u] = double2{0.0, 0.0};
-  conv_rows<0, (U - 1) * DMU + BP, S, U, DMU, BP, CT>(cur + (th.w * U * DMU + th.p * BP) * S + th.i,
+  for (int u = 0; u < U; ++u) acc[u] = C{0, 0};
+  conv_rows<0, (U - 1) * DMU + BP, S, U, DMU, BP, CT, C>(cur + (th.w * U * DMU + th.p * BP) * S + th.i,
                                                       th.h, acc);
   lds_barrier();
 
   // ---- tap-slice reduction (fixed order p = NP-1 .. 0) and post phase, into G = cur
-  double2 *G = cur;
+  C *G = cur;
   const int gbase = (th.w * U) * NT + th.tl;
 #pragma unroll
   for (int q = NP - 1; q >= 0; --q) {
     if (th.p == q) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        double2 a = acc[u];
-        double2 *gp = G + (gbase + u * NT) * C::GS + th.i;
+        C a = acc[u];
+        C *gp = G + (gbase + u * NT) * CF::GS + th.i;
         if (q != NP - 1) a = cadd(a, *gp);
         if (q == 0 && !CT) a = cmul(a, th.pf);
         *gp = a;
@@ -233,42 +235,43 @@ __device__ __forceinline__ void filter_tile(const FilterThread<BP, CT> &th, doub
   }
 
   // ---- S-point DFT of each row
-  sdft_passes<S, C::GR, C::GS, C::T, 0, 1>(G, tw, th.tid);
+  sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, C>(G, tw, th.tid);
 
   // ---- store segment-major: R = J*n_mu + theta (r fastest: runs of consecutive R)
 #pragma unroll
-  for (int c = 0; c < C::NSC; ++c) {
-    const int e = th.tid + c * C::T;
-    if ((C::GR * S) % C::T == 0 || e < C::GR * S) {
-      const int s = e / C::GR, r = e % C::GR;
+  for (int c = 0; c < CF::NSC; ++c) {
+    const int e = th.tid + c * CF::T;
+    if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
+      const int s = e / CF::GR, r = e % CF::GR;
       const long J = J0 + r / NT;
       if (J < nJ) {
         const long R = J * NMU + th.g * NT + r % NT;
-        at[(long)s * Mh + R] = G[r * C::GS + s];
+        at[(long)s * Mh + R] = G[r * CF::GS + s];
       }
     }
   }
-  if constexpr (C::DB) {
+  if constexpr (CF::DB) {
     // this wave's prefetch loads are complete once at most its stores are outstanding (the
     // stores were issued after them; a tile with a successor stores every row)
     asm volatile("" ::: "memory");
-    if (more) wait_vmcnt<C::NSC_MIN>();
+    if (more) wait_vmcnt<CF::NSC_MIN>();
   }
   lds_barrier();
 }
 
-template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
+template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C>
 __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
-                             (FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT>::MINW)) void k_filter(
-    const double2 *__restrict__ x, double2 *__restrict__ at, const double *__restrict__ taps,
-    const double2 *__restrict__ post, const double2 *__restrict__ twS, long M, long Mh, long nJ,
+                             (FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C>::MINW)) void k_filter(
+    const C *__restrict__ x, C *__restrict__ at, const real_t<C> *__restrict__ taps,
+    const C *__restrict__ post, const C *__restrict__ twS, long M, long Mh, long nJ,
     int tiles_per_block)
 {
-  typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT> C;
-  __shared__ double2 bufA[C::BUFN];
-  __shared__ double2 bufB[C::DB ? C::BUFN : 1];
-  __shared__ double2 tw[S];
-  FilterThread<BP, CT> th;
+  typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C> CF;
+  typedef real_t<C> RT;
+  __shared__ C bufA[CF::BUFN];
+  __shared__ C bufB[CF::DB ? CF::BUFN : 1];
+  __shared__ C tw[S];
+  FilterThread<BP, CT, C> th;
   th.tid = threadIdx.x;
   th.wave = th.tid >> 6;
   th.lane = th.tid & 63;
@@ -281,47 +284,47 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
 
   // phase group: the slow index of the block id so blocks sharing input rows are 8-apart
   // (same XCD under round-robin dispatch) and walk the same tiles at the same time
-  const int nblk_g = gridDim.x / C::NG;
+  const int nblk_g = gridDim.x / CF::NG;
   th.g = blockIdx.x / nblk_g;
   const int bg = blockIdx.x - th.g * nblk_g;
   const int theta = th.g * NT + th.tl;
   {
     const long base = ((long)theta * S + th.i) * (NP * BP) + th.p * BP;
     if constexpr (CT) {
-      const double2 *tp = reinterpret_cast<const double2 *>(taps) + base;
+      const C *tp = reinterpret_cast<const C *>(taps) + base;
 #pragma unroll
       for (int b = 0; b < BP; ++b) {
-        double2 v = tp[b];
+        C v = tp[b];
         th.h.re[b] = v.x;
         th.h.im[b] = v.y;
       }
     } else {
-      const double *tp = taps + base;
+      const RT *tp = taps + base;
 #pragma unroll
       for (int b = 0; b < BP; ++b) th.h.v[b] = tp[b];
     }
   }
   th.pf = post[theta * S + th.i];
-  for (int e = th.tid; e < S; e += C::T) tw[e] = twS[e];
+  for (int e = th.tid; e < S; e += CF::T) tw[e] = twS[e];
 
-  const long ntiles = (nJ + C::TJ - 1) / C::TJ;
+  const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
   const long t0 = (long)bg * tiles_per_block;
   const long t1 = min(ntiles, t0 + tiles_per_block);
   const long N = M * S;
 
-  if constexpr (C::DB) {
-    if (t0 < t1) dma_rows<C>(bufA, x, t0 * C::TJ * DMU * S, N, th.wave, th.lane);
+  if constexpr (CF::DB) {
+    if (t0 < t1) dma_rows<CF, C>(bufA, x, t0 * CF::TJ * DMU * S, N, th.wave, th.lane);
     wait_vmcnt<0>();
     lds_barrier();
     for (long tile = t0; tile < t1; ++tile) {
       const bool odd = (tile - t0) & 1;
-      filter_tile<C, S, NMU, NT, DMU, BP, U, NP, CT>(th, odd ? bufB : bufA, odd ? bufA : bufB,
+      filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, odd ? bufB : bufA, odd ? bufA : bufB,
                                                       tw, x, at, tile, tile + 1 < t1, N, Mh, nJ);
     }
   } else {
     lds_barrier();
     for (long tile = t0; tile < t1; ++tile)
-      filter_tile<C, S, NMU, NT, DMU, BP, U, NP, CT>(th, bufA, bufA, tw, x, at, tile, false, N,
+      filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, bufA, bufA, tw, x, at, tile, false, N,
                                                       Mh, nJ);
   }
 }
@@ -329,24 +332,33 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
 // ---------------------------------------------------------------------------------------------
 // Dispatch table
 // ---------------------------------------------------------------------------------------------
-template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
-static int launch(const SoiFilterArgs *a, hipStream_t st)
+template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C>
+static int launch1(const SoiFilterArgs *a, hipStream_t st)
 {
-  typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT> C;
-  static_assert(C::LDS_BYTES <= 160 * 1024, "filter tile exceeds LDS");
-  if ((long)C::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
-  auto kern = k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT>;
-  const long ntiles = (a->nJ + C::TJ - 1) / C::TJ;
+  typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C> CF;
+  static_assert(CF::LDS_BYTES <= 160 * 1024, "filter tile exceeds LDS");
+  if ((long)CF::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
+  auto kern = k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT, C>;
+  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
   // blocks per phase group: enough to fill the CUs, tiles split into contiguous ranges
-  long nb = 256L * C::BPC / C::NG;
+  long nb = 256L * CF::BPC / CF::NG;
   if (nb < 1) nb = 1;
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * C::NG)), dim3(C::T), 0, st,
-                     (const double2 *)a->x, (double2 *)a->at, a->taps, a->post,
-                     (const double2 *)a->twS, a->M, a->Mh, a->nJ, tpb);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * CF::NG)), dim3(CF::T), 0, st,
+                     (const C *)a->x, (C *)a->at, (const real_t<C> *)a->taps,
+                     (const C *)a->post, (const C *)a->twS, a->M, a->Mh, a->nJ, tpb);
   return (int)hipGetLastError();
+}
+
+// FP64 (the drop-in) or FP32 (the precision form: complex64 data, taps and twiddles rounded from
+// their FP64 values)
+template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT>
+static int launch(const SoiFilterArgs *a, hipStream_t st)
+{
+  if (a->f32) return launch1<S, NMU, NT, DMU, BP, U, NP, W, CT, float2>(a, st);
+  return launch1<S, NMU, NT, DMU, BP, U, NP, W, CT, double2>(a, st);
 }
 
 // Shape for (S, n_mu, d_mu, B, complex taps).  Returns 0 and fills NP/BP/NT/U/W, or -1.
@@ -408,11 +420,12 @@ extern "C" size_t soik_filter_lds_bytes(const SoiFilterArgs *a)
 {
   const int TJ = a->U * a->W;
   const int rows = TJ * a->d_mu + a->NP * a->BP - a->d_mu;
-  const long xn = ((long)rows * a->S + 63) / 64 * 64;
+  const int es = a->f32 ? 8 : 16, ch = 64 * (16 / es);
+  const long xn = ((long)rows * a->S + ch - 1) / ch * ch;
   const long gn = (long)TJ * a->NT * (a->S + 1);
   const long buf = xn > gn ? xn : gn;
-  const int db = 2 * buf * 16 + a->S * 16 <= 160 * 1024;
-  return (size_t)((db ? 2 : 1) * buf + a->S) * sizeof(double2);
+  const int db = (2 * buf + a->S) * es <= 160 * 1024;
+  return (size_t)((db ? 2 : 1) * buf + a->S) * es;
 }
 
 #define RT_CASE(SV, BPV, WV)                                                        \

@@ -20,13 +20,13 @@ typedef struct { float x, y; } soi_c32_t;
 #endif
 
 typedef struct SoiFilterArgs {
-  const void *x;      /* input [M][S], complex64/128 */
-  void *at;           /* output alpha_tilde [S][M_hat] */
-  const double *taps; /* [n_mu][S][NP*BP], real (ct = 0) or complex (ct = 1) */
-  const soi_c64_t *post;
-  const void *twS;    /* S-point twiddles */
+  const void *x;      /* input [M][S], complex128 (f32 = 0) or complex64 (f32 = 1) */
+  void *at;           /* output alpha_tilde [S][M_hat], same element type */
+  const void *taps;   /* [n_mu][S][NP*BP], real (ct = 0) or complex (ct = 1), double or float */
+  const void *post;   /* [n_mu][S] post phase, complex of the same precision */
+  const void *twS;    /* S-point twiddles, complex of the same precision */
   long M, Mh, nJ;
-  int S, n_mu, d_mu, BP, NP, NT, U, W, ct;
+  int S, n_mu, d_mu, BP, NP, NT, U, W, ct, f32;
 } SoiFilterArgs;
 
 int soik_make_tables(void *taps, soi_c64_t *post, soi_c64_t *w_full, soi_c64_t *winv, int S,
@@ -45,6 +45,14 @@ int soik_fft_col(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
 int soik_fft_row(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                  const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
                  hipStream_t st);
+int soik_fft_single_f32(const soi_c32_t *at, soi_c32_t *X, const soi_c32_t *tw,
+                        const soi_c64_t *winv, long L, long M, int S, int count, hipStream_t st);
+int soik_fft_col_f32(soi_c32_t *data, const soi_c32_t *tw1, const soi_c32_t *thi,
+                     const soi_c32_t *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
+                     hipStream_t st);
+int soik_fft_row_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
+                     const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
+                     hipStream_t st);
 int soik_fft_reg_supported(int L, int rows);
 int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                       const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
@@ -52,6 +60,12 @@ int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *th
 int soik_fft_rows_reg(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                       const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
                       hipStream_t st);
+int soik_fft_cols_reg_f32(soi_c32_t *data, const soi_c32_t *tw1, const soi_c32_t *thi,
+                          const soi_c32_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
+                          int layout, hipStream_t st);
+int soik_fft_rows_reg_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
+                          const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
+                          hipStream_t st);
 int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st);
 int soik_compress(int *out, long stride, const double *x, int len, int nchunk, int e_max,
                   const int *emi, int *nwords, hipStream_t st);

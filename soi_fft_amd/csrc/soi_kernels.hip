@@ -137,14 +137,14 @@ __global__ void k_f64_to_f32(const double2 *in, float2 *out, long n)
 // Step 3: M_hat-point FFT per segment, demodulation fused into the last pass
 // =============================================================================================
 // Single pass: `count` whole segments per block (M_hat small enough for LDS).
-__global__ __launch_bounds__(256) void k_fft_single_f64(const double2 *__restrict__ at,
-                                                         double2 *__restrict__ X,
-                                                         const double2 *__restrict__ tw,
-                                                         const double2 *__restrict__ winv,
-                                                         long L, long M, int S,
-                                                         int count)
+template <typename C>
+__global__ __launch_bounds__(256) void k_fft_single(const C *__restrict__ at, C *__restrict__ X,
+                                                    const C *__restrict__ tw,
+                                                    const double2 *__restrict__ winv, long L,
+                                                    long M, int S, int count)
 {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int s0 = blockIdx.x * count;
   const int nseq = min(count, S - s0);
@@ -157,20 +157,21 @@ __global__ __launch_bounds__(256) void k_fft_single_f64(const double2 *__restric
   lds_fft<32>(lds, (int)L, nseq, qs, 1, tw);
   for (int e = tid; e < nseq * M; e += nt) {
     int q = e / (int)M, m = e - q * (int)M;
-    X[(long)(s0 + q) * M + m] = cmul(winv[m], lds[q * qs + m]);
+    X[(long)(s0 + q) * M + m] = demod<C>(winv[m], lds[q * qs + m]);
   }
 }
 
 // Pass 1 (column FFTs): segment viewed as n1 x n2 row-major (R = n2*r1 + r2).  A block takes
 // Tc adjacent columns, runs n1-point FFTs down them, multiplies element m1 of column r2 by
 // e^{-2 pi i r2 m1 / M_hat} (two-level table: hi[e / B1] * lo[e % B1]) and stores in place.
-__global__ __launch_bounds__(256) void k_fft_col_f64(double2 *__restrict__ data,
-                                                      const double2 *__restrict__ tw1,
-                                                      const double2 *__restrict__ thi,
-                                                      const double2 *__restrict__ tlo, int n1, int n2, int Tc, int B1, long Mh,
-                                                      int S)
+template <typename C>
+__global__ __launch_bounds__(256) void k_fft_col(C *__restrict__ data, const C *__restrict__ tw1,
+                                                 const C *__restrict__ thi,
+                                                 const C *__restrict__ tlo, int n1, int n2,
+                                                 int Tc, int B1, long Mh, int S)
 {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int ncb = n2 / Tc;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void k_fft_col_f64(double2 *__restrict__ data,
   const int s = bid / ncb;
   if (s >= S) return;
   const int c0 = cb * Tc;
-  double2 *seg = data + (long)s * Mh;
+  C *seg = data + (long)s * Mh;
   for (int e = tid; e < n1 * Tc; e += nt) {
     int r1 = e / Tc, q = e - r1 * Tc;
     lds[e] = seg[(long)r1 * n2 + c0 + q];
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256) void k_fft_col_f64(double2 *__restrict__ data,
   for (int e = tid; e < n1 * Tc; e += nt) {
     int m1 = e / Tc, q = e - m1 * Tc;
     long ex = (long)(c0 + q) * m1;  // < M_hat
-    double2 t = cmul(thi[ex / B1], tlo[ex % B1]);
+    C t = cmul(thi[ex / B1], tlo[ex % B1]);
     seg[(long)m1 * n2 + c0 + q] = cmul(lds[e], t);
   }
 }
@@ -196,14 +197,14 @@ __global__ __launch_bounds__(256) void k_fft_col_f64(double2 *__restrict__ data,
 // Pass 2 (row FFTs + demodulation): a block takes Tr adjacent rows m1, runs n2-point FFTs along
 // them; element m2 of row m1 is y[m1 + n1*m2].  For m < M it stores W_inv[m] * y[m] to X[s*M+m]
 // (pfft.c:649-693), Tr adjacent m per contiguous run.
-__global__ __launch_bounds__(256) void k_fft_row_f64(const double2 *__restrict__ data,
-                                                      double2 *__restrict__ X,
-                                                      const double2 *__restrict__ tw2,
-                                                      const double2 *__restrict__ winv,
-                                                      int n1, int n2, int Tr, long Mh,
-                                                      long M, int S)
+template <typename C>
+__global__ __launch_bounds__(256) void k_fft_row(const C *__restrict__ data, C *__restrict__ X,
+                                                 const C *__restrict__ tw2,
+                                                 const double2 *__restrict__ winv, int n1,
+                                                 int n2, int Tr, long Mh, long M, int S)
 {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nrb = n1 / Tr;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -211,18 +212,18 @@ __global__ __launch_bounds__(256) void k_fft_row_f64(const double2 *__restrict__
   const int rb = bid / S;
   if (rb >= nrb) return;
   const int a = rb * Tr;
-  const double2 *seg = data + (long)s * Mh;
+  const C *seg = data + (long)s * Mh;
   for (int e = tid; e < n2 * Tr; e += nt) {
     int q = e / n2, r2 = e - q * n2;
     lds[r2 * Tr + q] = seg[(long)(a + q) * n2 + r2];
   }
   __syncthreads();
   lds_fft<32>(lds, n2, Tr, 1, Tr, tw2);
-  double2 *out = X + (long)s * M;
+  C *out = X + (long)s * M;
   for (int e = tid; e < n2 * Tr; e += nt) {
     int m2 = e / Tr, q = e - m2 * Tr;
     long m = (long)(a + q) + (long)n1 * m2;
-    if (m < M) out[m] = cmul(winv[m], lds[e]);
+    if (m < M) out[m] = demod<C>(winv[m], lds[e]);
   }
 }
 
@@ -501,42 +502,85 @@ static void set_lds_attr(const void *f)
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
+template <typename C>
+static int fft_single(const C *at, C *X, const C *tw, const double2 *winv, long L, long M, int S,
+                      int count, hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_single<C>); once = true; }
+  size_t lds = (size_t)count * (L + 1) * sizeof(C);
+  hipLaunchKernelGGL(k_fft_single<C>, dim3((S + count - 1) / count), dim3(256), lds, st, at, X,
+                     tw, winv, L, M, S, count);
+  return (int)hipGetLastError();
+}
+
+template <typename C>
+static int fft_col(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int Tc,
+                   int B1, long Mh, int S, hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_col<C>); once = true; }
+  size_t lds = (size_t)n1 * Tc * sizeof(C);
+  unsigned grid = (unsigned)((n2 / Tc) * S);
+  hipLaunchKernelGGL(k_fft_col<C>, dim3(grid), dim3(256), lds, st, data, tw1, thi, tlo, n1, n2,
+                     Tc, B1, Mh, S);
+  return (int)hipGetLastError();
+}
+
+template <typename C>
+static int fft_row(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2,
+                   int Tr, long Mh, long M, int S, hipStream_t st)
+{
+  static bool once = false;
+  if (!once) { set_lds_attr((const void *)k_fft_row<C>); once = true; }
+  size_t lds = (size_t)n2 * Tr * sizeof(C);
+  unsigned grid = (unsigned)((n1 / Tr) * S);
+  hipLaunchKernelGGL(k_fft_row<C>, dim3(grid), dim3(256), lds, st, data, X, tw2, winv, n1, n2,
+                     Tr, Mh, M, S);
+  return (int)hipGetLastError();
+}
+
 extern "C" int soik_fft_single(const double2 *at, double2 *X, const double2 *tw,
                                const double2 *winv, long L, long M, int S, int count,
                                hipStream_t st)
 {
-  static bool once = false;
-  if (!once) { set_lds_attr((const void *)k_fft_single_f64); once = true; }
-  size_t lds = (size_t)count * (L + 1) * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_single_f64, dim3((S + count - 1) / count), dim3(256), lds, st, at, X,
-                     tw, winv, L, M, S, count);
-  return (int)hipGetLastError();
+  return fft_single<double2>(at, X, tw, winv, L, M, S, count, st);
 }
 
 extern "C" int soik_fft_col(double2 *data, const double2 *tw1, const double2 *thi,
                             const double2 *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
                             hipStream_t st)
 {
-  static bool once = false;
-  if (!once) { set_lds_attr((const void *)k_fft_col_f64); once = true; }
-  size_t lds = (size_t)n1 * Tc * sizeof(double2);
-  unsigned grid = (unsigned)((n2 / Tc) * S);
-  hipLaunchKernelGGL(k_fft_col_f64, dim3(grid), dim3(256), lds, st, data, tw1, thi, tlo, n1, n2,
-                     Tc, B1, Mh, S);
-  return (int)hipGetLastError();
+  return fft_col<double2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, st);
 }
 
 extern "C" int soik_fft_row(const double2 *data, double2 *X, const double2 *tw2,
                             const double2 *winv, int n1, int n2, int Tr, long Mh, long M, int S,
                             hipStream_t st)
 {
-  static bool once = false;
-  if (!once) { set_lds_attr((const void *)k_fft_row_f64); once = true; }
-  size_t lds = (size_t)n2 * Tr * sizeof(double2);
-  unsigned grid = (unsigned)((n1 / Tr) * S);
-  hipLaunchKernelGGL(k_fft_row_f64, dim3(grid), dim3(256), lds, st, data, X, tw2, winv, n1, n2, Tr,
-                     Mh, M, S);
-  return (int)hipGetLastError();
+  return fft_row<double2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, st);
+}
+
+// FP32 form: complex64 data and tables; W_inv stays FP64
+extern "C" int soik_fft_single_f32(const float2 *at, float2 *X, const float2 *tw,
+                                   const double2 *winv, long L, long M, int S, int count,
+                                   hipStream_t st)
+{
+  return fft_single<float2>(at, X, tw, winv, L, M, S, count, st);
+}
+
+extern "C" int soik_fft_col_f32(float2 *data, const float2 *tw1, const float2 *thi,
+                                const float2 *tlo, int n1, int n2, int Tc, int B1, long Mh,
+                                int S, hipStream_t st)
+{
+  return fft_col<float2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, st);
+}
+
+extern "C" int soik_fft_row_f32(const float2 *data, float2 *X, const float2 *tw2,
+                                const double2 *winv, int n1, int n2, int Tr, long Mh, long M,
+                                int S, hipStream_t st)
+{
+  return fft_row<float2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, st);
 }
 
 extern "C" int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st)

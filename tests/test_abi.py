@@ -129,6 +129,8 @@ def test_defaults_match_reference():
     (dict(rank=1), 1, 1),                    # one GPU runs every rank
     (dict(), 2000, 1),                       # k <= 1024 (pfft.c:274)
     (dict(B=120), 1, 1),                     # taps beyond the supported 80
+    (dict(precision=1, use_vlc=1), 1, 1),    # VLC words encode FP64 values (compress.c)
+    (dict(precision=7), 1, 1),               # unknown precision
 ])
 def test_init_rejects(kw, k, status):
     d = _desc(**kw)

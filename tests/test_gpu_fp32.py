@@ -1,0 +1,98 @@
+"""The FP32 precision form (precision = SOI_PREC_FP32): complex64 input and output, FP32 filter
+contraction and FFT arithmetic, windows / twiddles generated in FP64 and W_inv applied in FP64.
+
+Error is stated against the FP64 transform of the same complex64 input (the oracle, or rocFFT's
+FP64 FFT at N = 2^27), so input rounding is not counted.  Bound: relative L2 <= 1e-6 for the
+default window (max|W_inv| = 9.06); narrower windows amplify FP32 rounding near segment edges
+by their demodulation gain G = max|W_inv|, bounded by 2^-24 * G."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_util import rel
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-6
+
+
+def fp32_tol(orc, p):
+    gain = float(np.abs(orc.w_inv_table(p)).max())
+    return max(FP32_TOL, 2.0 ** -24 * gain)
+
+
+def run_soi32(soi, torch, x32, P, k=1, **kw):
+    d = soi.make_descriptor(x32.size, P, k, precision=soi.SOI_PREC_FP32, **kw)
+    try:
+        xt = torch.from_numpy(np.ascontiguousarray(x32)).cuda()
+        torch.cuda.synchronize()
+        soi.compute_soi(d, xt.data_ptr())
+        return xt.cpu().numpy()
+    finally:
+        soi.free_soi_descriptor(d)
+
+
+SHAPES32 = [
+    (2 ** 13, 4, 1, {}),
+    (2 ** 16, 8, 1, {}),
+    (2 ** 16, 4, 2, {}),
+    (2 ** 20, 8, 1, {}),
+    (5 * 2 ** 14, 8, 1, {}),
+    (2 ** 18, 32, 1, {}),
+    (2 ** 17, 8, 1, dict(B=64, tau=818 / 1024, sigma=267.4513)),
+    (7 * 2 ** 15, 8, 1, dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)),
+]
+
+
+@pytest.mark.parametrize("N,P,k,kw", SHAPES32)
+def test_fp32_form_matches_fp64_oracle(soi, orc, torch_cuda, N, P, k, kw):
+    print(f"case N={N} P={P} k={k} {kw}", flush=True)
+    x32 = orc.uniform_input(N, 202).astype(np.complex64)
+    got = run_soi32(soi, torch_cuda, x32, P, k, **kw)
+    assert got.dtype == np.complex64
+    okw = {kk: v for kk, v in kw.items() if kk in ("n_mu", "d_mu", "B", "tau", "sigma")}
+    p = orc.params(N, P, k=k, **okw)
+    ref, _ = orc.compute_soi(p, x32.astype(np.complex128))
+    err = rel(got.astype(np.complex128), ref)
+    print(f"  fp32 rel-L2 vs fp64 oracle {err:.3e}", flush=True)
+    assert err < fp32_tol(orc, p)
+
+
+def test_fp32_host_pointer_path(soi, orc, torch_cuda):
+    N, P = 2 ** 16, 8
+    x32 = orc.uniform_input(N, 7).astype(np.complex64)
+    d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    try:
+        buf = x32.copy()
+        soi.compute_soi(d, buf.ctypes.data)
+    finally:
+        soi.free_soi_descriptor(d)
+    ref, _ = orc.compute_soi(orc.params(N, P), x32.astype(np.complex128))
+    assert rel(buf.astype(np.complex128), ref) < FP32_TOL
+
+
+def test_fp32_full_size_vs_rocfft(soi, torch_cuda):
+    """N = 2^27, P = 32: the tone + noise input cast to complex64, against rocFFT's FP64 FFT of
+    the cast values."""
+    torch = torch_cuda
+    N, P = 2 ** 27, 32
+    rng = np.random.default_rng(2027)
+    f = np.floor(rng.uniform(0, N, 128)) + rng.uniform(0, 1, 128)
+    ph = rng.uniform(0, 1, 128)
+    x64 = torch.empty(N, dtype=torch.complex128, device="cuda")
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert soi.lib().soi_fill_tones(x64.data_ptr(), N, f.ctypes.data_as(dp), ph.ctypes.data_as(dp),
+                                    len(f), 1.0, 1e-3, 2027, None) == 0
+    x32 = x64.to(torch.complex64)
+    ref = torch.fft.fft(x32.to(torch.complex128))
+    del x64
+    d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    try:
+        torch.cuda.synchronize()
+        soi.compute_soi(d, x32.data_ptr())
+    finally:
+        soi.free_soi_descriptor(d)
+    err = (torch.linalg.norm(x32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+    print("N=2^27 fp32 rel-L2 vs rocFFT fp64", err, flush=True)
+    assert err < FP32_TOL

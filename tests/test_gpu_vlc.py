@@ -141,3 +141,46 @@ def test_soi_vlc_matches_oracle(soi, orc, torch_cuda, N, P, kw):
     assert seg == oseg and tot == max(oseg)
     sc, _ = orc.vlc_plan(P, 1, at.shape[1] // P, at.shape[1], 1.0, oseg)
     assert cnt == list(sc)
+
+
+def test_soi_vlc_banded_full_size(soi, orc, torch_cuda):
+    """The banded VLC workload at full size: N = 7*2^23 complex128, P = 64, mu = 8/7, B = 72;
+    256 unit tones inside one band of width N/64 plus white noise at -60 dB.  The VLC run must
+    agree with the uncompressed run to the FP64 parity bound (the codec keeps 55 - Delta bits
+    per value against the global maximum exponent), and both with rocFFT's FP64 FFT to the
+    window's accuracy."""
+    torch = torch_cuda
+    N, P = 7 * 2 ** 23, 64
+    kw = dict(n_mu=8, d_mu=7, tau=0.7, sigma=360.0)
+    rng = np.random.default_rng(5)
+    lo = int(rng.integers(0, P)) * (N // P)
+    f = lo + rng.uniform(0, N // P, 256)
+    ph = rng.uniform(0, 1, 256)
+    sigma = float(np.sqrt(256 * 1e-6 / 2))
+    x = torch.empty(N, dtype=torch.complex128, device="cuda")
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert soi.lib().soi_fill_tones(x.data_ptr(), N, f.ctypes.data_as(dp), ph.ctypes.data_as(dp),
+                                    len(f), 1.0, sigma, 5, None) == 0
+    ref = torch.fft.fft(x)
+    outs = {}
+    for vlc in (False, True):
+        y = x.clone()
+        d = soi.make_descriptor(N, P, use_vlc=vlc, **kw)
+        try:
+            torch.cuda.synchronize()
+            soi.compute_soi(d, y.data_ptr())
+            if vlc:
+                sb = list(d.segmentBoundaries[: P + 1])
+                seg, tot, cnt = soi.vlc_exponents(d)
+        finally:
+            soi.free_soi_descriptor(d)
+        outs[vlc] = y
+    e0 = (torch.linalg.norm(outs[False] - ref) / torch.linalg.norm(ref)).item()
+    ev = (torch.linalg.norm(outs[True] - outs[False]) / torch.linalg.norm(outs[False])).item()
+    print(f"banded N=7*2^23: rel-L2 vs rocFFT {e0:.3e}, vlc vs plain {ev:.3e}", flush=True)
+    gain = float(np.abs(orc.w_inv_table(orc.params(7 * 2 ** 13, 4, **kw))).max())
+    assert e0 < 1e-10
+    assert ev < max(1e-12, 30 * 2.0 ** -52 * gain)
+    # segment boundaries: a partition of the S segments over the P ranks
+    assert sb[0] == 0 and sb[-1] == P and all(b <= c for b, c in zip(sb, sb[1:]))
+    assert tot == max(seg) and len(cnt) == P

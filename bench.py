@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-sample-log2n", type=int, default=25)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=512)
     return ap.parse_args()
 
 
@@ -80,6 +82,18 @@ def fft_split(Mh):
     return best if best else (Mh, 1)
 
 
+def step_buffers(torch, x, steps):
+    """Up to 8 device copies of the input; timed step i transforms copy i % 8 in place, so a
+    copy is transformed at most ceil(steps / 8) times (each transform scales the norm by
+    sqrt(N), which a complex64 copy tolerates a few times over before overflowing)."""
+    return [x.clone() for _ in range(max(1, min(steps, 8)))]
+
+
+def refill(torch, bufs, x):
+    for b in bufs:
+        b.copy_(x)
+
+
 def bench_fp32(args, soi, torch, x64, N, P, fp64_gflops):
     """The FP32 precision form on the same input rounded to complex64: throughput over the same
     number of timed steps, and relative L2 error against rocFFT's FP64 FFT of the rounded input
@@ -94,19 +108,22 @@ def bench_fp32(args, soi, torch, x64, N, P, fp64_gflops):
     err = (torch.linalg.norm(y.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
     del ref, y
     torch.cuda.empty_cache()
+    bufs = step_buffers(torch, x, args.steps)
     for _ in range(args.warmup):
-        soi.compute_soi_async(d, x.data_ptr(), None)
+        soi.compute_soi_async(d, bufs[0].data_ptr(), None)
+    refill(torch, bufs, x)
     torch.cuda.synchronize()
     soi.enable_timing(d, True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for _ in range(args.steps):
-        soi.compute_soi_async(d, x.data_ptr(), None)
+    for i in range(args.steps):
+        soi.compute_soi_async(d, bufs[i % len(bufs)].data_ptr(), None)
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
     acc = soi.stage_times(d)
+    del bufs
     soi.enable_timing(d, False)
     soi.free_soi_descriptor(d)
     gflops = 5.0 * N * math.log2(N) / (ms * 1e-3) / 1e9
@@ -117,6 +134,55 @@ def bench_fp32(args, soi, torch, x64, N, P, fp64_gflops):
                        "fft_pass2_demod_ms": round(acc[3], 4)},
             "form": "complex64 data; FP32 filter contraction (VALU) and FFTs; FP64 window, "
                     "twiddle and W_inv generation; W_inv applied in FP64"}
+
+
+def bench_batch(args, soi, torch, prec):
+    """The batching form: args.batch_count independent transforms, N uniform over {2^16, 3*2^15,
+    2^18, 5*2^16, 2^20}, P uniform over {8, 16, 32} (rejecting (2^16, 32), whose M/P < B),
+    uniform[-1,1] inputs, captured into one HIP graph (each shape autotuned once) and replayed:
+    one step = one graph launch transforming all of them."""
+    import numpy as np
+    sizes, ps = [2 ** 16, 3 * 2 ** 15, 2 ** 18, 5 * 2 ** 16, 2 ** 20], [8, 16, 32]
+    rng = np.random.default_rng(2028)
+    shapes = []
+    while len(shapes) < args.batch_count:
+        n, p = int(rng.choice(sizes)), int(rng.choice(ps))
+        if (n, p) != (2 ** 16, 32):
+            shapes.append((n, p))
+    cdt = torch.complex64 if prec else torch.complex128
+    descs, bufs = [], []
+    t0 = time.perf_counter()
+    for i, (n, p) in enumerate(shapes):
+        descs.append(soi.make_descriptor(n, p, precision=prec))
+        b = torch.empty(n, dtype=torch.complex128, device="cuda")
+        soi.lib().soi_fill_uniform(b.data_ptr(), n, 7000 + i, None)
+        bufs.append(b.to(cdt) if prec else b)
+    torch.cuda.synchronize()
+    batch = soi.Batch(descs, [b.data_ptr() for b in bufs], autotune=True)
+    setup_s = time.perf_counter() - t0
+    stream = torch.cuda.ExternalStream(batch.stream)
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        batch.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    flop = sum(5.0 * n * math.log2(n) for n, _ in shapes)
+    batch.close()
+    for d in descs:
+        soi.free_soi_descriptor(d)
+    del bufs
+    torch.cuda.empty_cache()
+    return {"value": round(flop / (ms * 1e-3) / 1e9, 2), "unit": "GFLOP/s",
+            "ms_per_step": round(ms, 4), "transforms": len(shapes),
+            "points": int(sum(n for n, _ in shapes)), "setup_s": round(setup_s, 2),
+            "dtype": "f32" if prec else "f64",
+            "launch": "one HIP graph launch per step (4 concurrent capture branches)"}
 
 
 def cpu_baseline(log2n, P):
@@ -169,16 +235,18 @@ def main():
     del ref, y
     torch.cuda.empty_cache()
 
+    bufs = step_buffers(torch, x, args.steps)
     for _ in range(args.warmup):
-        soi.compute_soi_async(d, x.data_ptr(), None)
+        soi.compute_soi_async(d, bufs[0].data_ptr(), None)
+    refill(torch, bufs, x)
     torch.cuda.synchronize()
     soi.enable_timing(d, True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(stream)
-    for _ in range(args.steps):
-        soi.compute_soi_async(d, x.data_ptr(), None)
+    for i in range(args.steps):
+        soi.compute_soi_async(d, bufs[i % len(bufs)].data_ptr(), None)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -187,6 +255,7 @@ def main():
     gflops = flop / (ms * 1e-3) / 1e9
     # mean per-kernel durations over the timed steps (the last 64 when steps > 64)
     acc = soi.stage_times(d)
+    del bufs
     soi.enable_timing(d, False)
     S, M, Mh = d.S, d.M, d.M_hat
     soi.free_soi_descriptor(d)
@@ -241,6 +310,11 @@ def main():
     }
     if not args.no_fp32:
         out["fp32_split"] = bench_fp32(args, soi, torch, x, N, P, gflops)
+    if not args.no_batch:
+        del x
+        torch.cuda.empty_cache()
+        out["batch_form"] = {"fp64": bench_batch(args, soi, torch, 0),
+                             "fp32": bench_batch(args, soi, torch, 1)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2n, P)
     print(json.dumps(out))

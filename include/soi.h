@@ -158,6 +158,27 @@ int soi_stage_times(soi_desc_t *d, double *ms, int max_stages);
  * (pfft.c:319-343), and the per-segment send counts in doubles (pfft.c:404). */
 int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts);
 
+/* ---- FFT tuning ----------------------------------------------------------------------------
+ * The segment FFT of a plan has one or more equivalent kernel configurations (tile widths and
+ * register layouts).  soi_fft_variants returns their number, soi_set_fft_variant selects one,
+ * and soi_autotune times each on `scratch` (a device buffer of N elements whose contents are
+ * overwritten) over `reps` calls, keeps the fastest and returns its index (-1 on error). */
+int soi_fft_variants(soi_desc_t *d);
+int soi_set_fft_variant(soi_desc_t *d, int variant);
+int soi_autotune(soi_desc_t *d, cfft_complex_t *scratch, int reps);
+
+/* ---- batching form ----------------------------------------------------------------------
+ * Many independent transforms (initialised descriptors of any shapes and either precision,
+ * without use_vlc), each with its own device buffer, captured once into one HIP graph whose
+ * branches run concurrently; soi_batch_run replays it with one launch (in place on the same
+ * buffers, which the caller refills between runs).  With autotune != 0 each distinct shape is
+ * tuned once (soi_autotune) before capture.  Returns NULL on error (soi_last_error). */
+typedef struct soi_batch_s soi_batch_t;
+soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int count, int autotune);
+int soi_batch_run(soi_batch_t *b, void *stream);   /* NULL = the batch's own stream */
+void *soi_batch_stream(soi_batch_t *b);
+void soi_batch_destroy(soi_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif

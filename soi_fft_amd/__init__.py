@@ -124,6 +124,20 @@ def lib():
         L.soi_fill_tones.argtypes = [vp, ctypes.c_long, dp, dp, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_uint64, vp]
         L.soi_fill_tones.restype = ctypes.c_int
+        L.soi_fft_variants.argtypes = [P]
+        L.soi_fft_variants.restype = ctypes.c_int
+        L.soi_set_fft_variant.argtypes = [P, ctypes.c_int]
+        L.soi_set_fft_variant.restype = ctypes.c_int
+        L.soi_autotune.argtypes = [P, vp, ctypes.c_int]
+        L.soi_autotune.restype = ctypes.c_int
+        L.soi_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(vp), ctypes.c_int,
+                                       ctypes.c_int]
+        L.soi_batch_create.restype = vp
+        L.soi_batch_run.argtypes = [vp, vp]
+        L.soi_batch_run.restype = ctypes.c_int
+        L.soi_batch_stream.argtypes = [vp]
+        L.soi_batch_stream.restype = vp
+        L.soi_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -192,6 +206,56 @@ def vlc_exponents(d):
     if n == 0:
         return None
     return list(seg), tot.value, list(cnt)
+
+
+def fft_variants(d):
+    n = lib().soi_fft_variants(ctypes.byref(d))
+    _check()
+    return n
+
+
+def set_fft_variant(d, v):
+    lib().soi_set_fft_variant(ctypes.byref(d), v)
+    _check()
+
+
+def autotune(d, scratch_ptr, reps=3):
+    """Time each FFT variant on the device buffer at scratch_ptr (overwritten); keep the best."""
+    v = lib().soi_autotune(ctypes.byref(d), ctypes.c_void_p(scratch_ptr), reps)
+    _check()
+    return v
+
+
+class Batch:
+    """Batching form: initialised descriptors + their device buffers, captured into one HIP
+    graph (soi_batch_create); run() replays it with one launch, in place on the buffers."""
+
+    def __init__(self, descs, ptrs, autotune=True):
+        n = len(descs)
+        P = ctypes.POINTER(SoiDesc)
+        self._descs = descs
+        arr = (P * n)(*[ctypes.pointer(d) for d in descs])
+        bufs = (ctypes.c_void_p * n)(*ptrs)
+        self._h = lib().soi_batch_create(arr, bufs, n, int(autotune))
+        _check()
+        if not self._h:
+            raise SoiError(lib().soi_last_status(), lib().soi_last_error().decode())
+
+    def run(self, stream=None):
+        lib().soi_batch_run(self._h, ctypes.c_void_p(stream))
+        _check()
+
+    @property
+    def stream(self):
+        return lib().soi_batch_stream(self._h)
+
+    def close(self):
+        if self._h:
+            lib().soi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def make_descriptor(N, P, k=1, n_mu=5, d_mu=4, B=72, tau=None, sigma=None, use_vlc=False,

@@ -87,6 +87,9 @@ typedef struct {
   /* fft */
   int fft_single, single_count, fft_reg;
   int n1, n2, Tc, Tr, B1, lb1, clay;
+  /* register-FFT alternatives (column tile, column variant, row tile) and the one in use */
+  int nvar, var;
+  int vtab[16][3];
   soi_c64_t *tw1, *tw2, *thi, *tlo;
   /* buffers */
   soi_c64_t *io;        /* staging for host-pointer inputs */
@@ -184,6 +187,32 @@ static int pick_tile(int n, long other, int cap)
   return 0;
 }
 
+/* Register-FFT alternatives that exist for (n1, n2): column tile 8 or 4 with its kernel
+ * variant, row tile 8 or 4.  The planned choice is entry 0 of the table. */
+static void fft_variants(soi_plan_t *pl)
+{
+  int cols[4][2], nc = 0, rows[2], nr = 0;
+  const int n1 = pl->n1, n2 = pl->n2;
+  const int c4 = n1 == 1024 || n1 == 1280 || n1 == 2048 || n1 == 2560;
+  cols[nc][0] = pl->Tc, cols[nc][1] = pl->clay, ++nc;
+  if (n2 % 8 == 0 && !(pl->Tc == 8 && pl->clay == 0)) cols[nc][0] = 8, cols[nc][1] = 0, ++nc;
+  if (c4 && n2 % 4 == 0 && !(pl->Tc == 4 && pl->clay == 0)) cols[nc][0] = 4, cols[nc][1] = 0, ++nc;
+  if ((n1 == 2048 || n1 == 2560) && n2 % 8 == 0 && pl->clay != 1)
+    cols[nc][0] = 8, cols[nc][1] = 1, ++nc;
+  if (n1 == 2560 && n2 % 4 == 0 && pl->clay != 2 && nc < 4) cols[nc][0] = 4, cols[nc][1] = 2, ++nc;
+  rows[nr++] = pl->Tr;
+  if (pl->Tr != 8 && n2 != 2560 && n1 % 8 == 0) rows[nr++] = 8;
+  if (pl->Tr != 4 && (n2 == 1024 || n2 == 1280 || n2 == 2048 || n2 == 2560) && n1 % 4 == 0)
+    rows[nr++] = 4;
+  pl->nvar = 0;
+  for (int r = 0; r < nr; ++r)
+    for (int c = 0; c < nc; ++c) {
+      int *v = pl->vtab[pl->nvar++];
+      v[0] = cols[c][0], v[1] = cols[c][1], v[2] = rows[r];
+    }
+  pl->var = 0;
+}
+
 static int plan_fft(soi_plan_t *pl)
 {
   const long Mh = pl->Mh;
@@ -193,6 +222,8 @@ static int plan_fft(soi_plan_t *pl)
     return -1;
   }
   if (Mh + 1 <= CAP) {
+    pl->nvar = 1;
+    pl->var = 0;
     pl->fft_single = 1;
     pl->single_count = (int)(CAP / (Mh + 1));
     if (pl->single_count > pl->S) pl->single_count = (int)pl->S;
@@ -246,6 +277,7 @@ static int plan_fft(soi_plan_t *pl)
       while ((1L << lb) * (1L << lb) < Mh) ++lb;
       pl->B1 = 1 << lb;
       pl->lb1 = lb;
+      fft_variants(pl);
       return 0;
     }
   }
@@ -278,6 +310,9 @@ static int plan_fft(soi_plan_t *pl)
   pl->n2 = bn2;
   pl->Tc = bTc;
   pl->Tr = bTr;
+  pl->nvar = 1;
+  pl->var = 0;
+  pl->vtab[0][0] = bTc, pl->vtab[0][1] = 0, pl->vtab[0][2] = bTr;
   int b1 = 1, lb = 0;
   while ((long)b1 * b1 < Mh) b1 <<= 1, ++lb;
   pl->B1 = b1;
@@ -594,7 +629,7 @@ static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, hipStream_t 
 static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
 {
   const int S = (int)pl->S, P = pl->P;
-  const long l = pl->l, Mh = pl->Mh;
+  const long l = pl->l;
   const int nch = S * P;
   double *at = (double *)d->alpha_tilde;
   /* chunk (s, p) = alpha_tilde[s][p*l .. (p+1)*l): 2l doubles at at + 2*(s*Mh + p*l);
@@ -799,6 +834,73 @@ fail:
   return 0;
 }
 
+int soi_fft_variants(soi_desc_t *d)
+{
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised (call init_soi_descriptor)");
+    return 0;
+  }
+  return pl->nvar;
+}
+
+int soi_set_fft_variant(soi_desc_t *d, int v)
+{
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised (call init_soi_descriptor)");
+    return -1;
+  }
+  if (v < 0 || v >= pl->nvar) {
+    set_err(SOI_ERR_ARGS, "FFT variant %d out of range [0, %d)", v, pl->nvar);
+    return -1;
+  }
+  pl->var = v;
+  if (pl->fft_reg) {
+    pl->Tc = pl->vtab[v][0];
+    pl->clay = pl->vtab[v][1];
+    pl->Tr = pl->vtab[v][2];
+  }
+  return 0;
+}
+
+int soi_autotune(soi_desc_t *d, cfft_complex_t *scratch, int reps)
+{
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised (call init_soi_descriptor)");
+    return -1;
+  }
+  if (!scratch || !is_device_ptr(scratch)) {
+    set_err(SOI_ERR_ARGS, "autotune needs a device scratch buffer of N elements");
+    return -1;
+  }
+  if (reps < 1) reps = 3;
+  const int was_timing = pl->timing;
+  int best = 0;
+  double best_ms = 1e300;
+  for (int v = 0; v < pl->nvar; ++v) {
+    if (soi_set_fft_variant(d, v)) return -1;
+    compute_impl(d, scratch, NULL, 1, 0); /* warm-up (first launch sets kernel attributes) */
+    if (soi_last_status()) return -1;
+    soi_enable_timing(d, 1);
+    for (int r = 0; r < reps; ++r) {
+      compute_impl(d, scratch, NULL, 0, 0);
+      if (soi_last_status()) return -1;
+    }
+    double ms[NSTAGE];
+    int n = soi_stage_times(d, ms, NSTAGE);
+    double t = n == NSTAGE ? ms[2] + ms[3] : 1e300;
+    if (t < best_ms) best_ms = t, best = v;
+  }
+  soi_enable_timing(d, was_timing);
+  if (soi_set_fft_variant(d, best)) return -1;
+  return best;
+}
+
 int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts)
 {
   soi_plan_t *pl = (soi_plan_t *)d->plan;
@@ -838,3 +940,153 @@ int soi_fill_tones(void *x, long N, const double *freqs, const double *phases, i
   (void)hipFree(dp);
   return e;
 }
+
+/* ---- batching form: many initialised descriptors replayed as one HIP graph ---------------- */
+
+#define SOI_BATCH_BRANCHES 4 /* independent capture branches (the device's hardware queues) */
+
+struct soi_batch_s {
+  int count;
+  int device;
+  hipStream_t stream;
+  hipStream_t branch[SOI_BATCH_BRANCHES];
+  hipEvent_t fork, join[SOI_BATCH_BRANCHES];
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+static int same_shape(const soi_desc_t *a, const soi_desc_t *b)
+{
+  return a->N == b->N && a->P == b->P && a->k == b->k && a->precision == b->precision &&
+         a->n_mu == b->n_mu && a->d_mu == b->d_mu && a->B == b->B && a->tau == b->tau &&
+         a->sigma == b->sigma && a->window_mode == b->window_mode;
+}
+
+void soi_batch_destroy(soi_batch_t *b)
+{
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+    if (b->branch[r]) (void)hipStreamDestroy(b->branch[r]);
+    if (b->join[r]) (void)hipEventDestroy(b->join[r]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  free(b);
+}
+
+soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int count, int autotune)
+{
+  clear_err();
+  soi_batch_t *b = NULL;
+  int capturing = 0;
+  if (!descs || !bufs || count < 1) {
+    set_err(SOI_ERR_ARGS, "soi_batch_create needs count >= 1 descriptors and buffers");
+    return NULL;
+  }
+  int device = -1;
+  for (int i = 0; i < count; ++i) {
+    soi_plan_t *pl = descs[i] ? (soi_plan_t *)descs[i]->plan : NULL;
+    if (!pl) {
+      set_err(SOI_ERR_STATE, "batch entry %d: descriptor is not initialised", i);
+      return NULL;
+    }
+    if (descs[i]->use_vlc) {
+      set_err(SOI_ERR_ARGS, "batch entry %d: use_vlc needs host round trips and cannot be "
+              "captured in a graph", i);
+      return NULL;
+    }
+    if (!bufs[i] || !is_device_ptr(bufs[i])) {
+      set_err(SOI_ERR_ARGS, "batch entry %d: buffer must be a device pointer", i);
+      return NULL;
+    }
+    if (device >= 0 && pl->device != device) {
+      set_err(SOI_ERR_ARGS, "batch entry %d: all descriptors must belong to one device", i);
+      return NULL;
+    }
+    device = pl->device;
+  }
+  HIPCHK(hipSetDevice(device));
+  /* autotune: one timing sweep per distinct shape, on a scratch buffer */
+  if (autotune) {
+    for (int i = 0; i < count; ++i) {
+      int j = 0;
+      while (j < i && !same_shape(descs[j], descs[i])) ++j;
+      if (j < i) {
+        soi_plan_t *pj = (soi_plan_t *)descs[j]->plan;
+        if (soi_set_fft_variant(descs[i], pj->var)) return NULL;
+        continue;
+      }
+      soi_plan_t *pl = (soi_plan_t *)descs[i]->plan;
+      size_t bytes = (size_t)pl->N * (pl->f32 ? sizeof(soi_c32_t) : sizeof(soi_c64_t));
+      void *scratch = NULL;
+      if (hipMalloc(&scratch, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_err(SOI_ERR_ALLOC, "autotune scratch of %zu bytes", bytes);
+        return NULL;
+      }
+      (void)hipMemset(scratch, 0, bytes);
+      int r = soi_autotune(descs[i], (cfft_complex_t *)scratch, 3);
+      (void)hipFree(scratch);
+      if (r < 0) return NULL;
+    }
+  }
+  b = (soi_batch_t *)calloc(1, sizeof(*b));
+  if (!b) {
+    set_err(SOI_ERR_ALLOC, "host allocation failed");
+    return NULL;
+  }
+  b->count = count;
+  b->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+    HIPCHK(hipStreamCreateWithFlags(&b->branch[r], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming));
+  }
+  /* timing events would become graph nodes: capture with timing off */
+  for (int i = 0; i < count; ++i) ((soi_plan_t *)descs[i]->plan)->timing = 0;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  capturing = 1;
+  HIPCHK(hipEventRecord(b->fork, b->stream));
+  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) HIPCHK(hipStreamWaitEvent(b->branch[r], b->fork, 0));
+  for (int i = 0; i < count; ++i) {
+    compute_impl(descs[i], bufs[i], b->branch[i % SOI_BATCH_BRANCHES], 0, 0);
+    if (soi_last_status()) goto fail_quiet;
+  }
+  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+    HIPCHK(hipEventRecord(b->join[r], b->branch[r]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->join[r], 0));
+  }
+  capturing = 0;
+  HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+  return b;
+fail:
+fail_quiet:
+  if (capturing) {
+    hipGraph_t g = NULL;
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+  }
+  soi_batch_destroy(b);
+  return NULL;
+}
+
+int soi_batch_run(soi_batch_t *b, void *stream)
+{
+  clear_err();
+  if (!b || !b->exec) {
+    set_err(SOI_ERR_STATE, "batch is not created");
+    return SOI_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->stream));
+  return 0;
+fail:
+  return soi_last_status();
+}
+
+void *soi_batch_stream(soi_batch_t *b) { return b ? (void *)b->stream : NULL; }

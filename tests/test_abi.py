@@ -146,3 +146,19 @@ def test_state_errors():
     assert e.value.status == 6
     with pytest.raises(soi.SoiError):
         soi.free_soi_descriptor(d)
+
+
+def test_batch_rejects_bad_arguments():
+    """soi_batch_create validates before any device work: an uninitialised descriptor is a state
+    error, an empty batch an argument error."""
+    import ctypes
+    L = soi.lib()
+    d = _desc()
+    P = ctypes.POINTER(soi.SoiDesc)
+    arr = (P * 1)(ctypes.pointer(d))
+    bufs = (ctypes.c_void_p * 1)(None)
+    assert not L.soi_batch_create(arr, bufs, 1, 0)
+    assert L.soi_last_status() == 6
+    assert not L.soi_batch_create(arr, bufs, 0, 0)
+    assert L.soi_last_status() == 1
+    assert L.soi_fft_variants(ctypes.byref(d)) == 0 and L.soi_last_status() == 6

@@ -31,6 +31,15 @@ template <typename C> __device__ __forceinline__ C demod(double2 w, C y)
   const double yx = (double)y.x, yy = (double)y.y;
   return C{(real_t<C>)(w.x * yx - w.y * yy), (real_t<C>)(w.x * yy + w.y * yx)};
 }
+// Streaming store of one complex value (one 8- or 16-byte non-temporal vector store): for
+// outputs that are not read again before they leave the caches
+template <typename C> __device__ __forceinline__ void store_nt(C *p, C v)
+{
+  typedef real_t<C> R;
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  v2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<v2 *>(p));
+}
 // multiply by -i
 template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return C{a.y, -a.x}; }
 template <typename C, typename S> __device__ __forceinline__ C cscale(C a, S s) { return C{a.x * s, a.y * s}; }

@@ -246,7 +246,7 @@ __device__ __forceinline__ void filter_tile(const FilterThread<BP, CT, C> &th, C
       const long J = J0 + r / NT;
       if (J < nJ) {
         const long R = J * NMU + th.g * NT + r % NT;
-        at[(long)s * Mh + R] = G[r * CF::GS + s];
+        store_nt(at + (long)s * Mh + R, G[r * CF::GS + s]);
       }
     }
   }

@@ -191,7 +191,7 @@ static int pick_tile(int n, long other, int cap)
  * variant, row tile 8 or 4.  The planned choice is entry 0 of the table. */
 static void fft_variants(soi_plan_t *pl)
 {
-  int cols[4][2], nc = 0, rows[2], nr = 0;
+  int cols[5][2], nc = 0, rows[2], nr = 0;
   const int n1 = pl->n1, n2 = pl->n2;
   const int c4 = n1 == 1024 || n1 == 1280 || n1 == 2048 || n1 == 2560;
   cols[nc][0] = pl->Tc, cols[nc][1] = pl->clay, ++nc;
@@ -199,7 +199,9 @@ static void fft_variants(soi_plan_t *pl)
   if (c4 && n2 % 4 == 0 && !(pl->Tc == 4 && pl->clay == 0)) cols[nc][0] = 4, cols[nc][1] = 0, ++nc;
   if ((n1 == 2048 || n1 == 2560) && n2 % 8 == 0 && pl->clay != 1)
     cols[nc][0] = 8, cols[nc][1] = 1, ++nc;
-  if (n1 == 2560 && n2 % 4 == 0 && pl->clay != 2 && nc < 4) cols[nc][0] = 4, cols[nc][1] = 2, ++nc;
+  if (n1 == 2560 && n2 % 4 == 0 && pl->clay != 2 && nc < 5) cols[nc][0] = 4, cols[nc][1] = 2, ++nc;
+  if (n1 == 2560 && pl->f32 && n2 % 8 == 0 && pl->clay != 3 && nc < 5)
+    cols[nc][0] = 8, cols[nc][1] = 3, ++nc;
   rows[nr++] = pl->Tr;
   if (pl->Tr != 8 && n2 != 2560 && n1 % 8 == 0) rows[nr++] = 8;
   if (pl->Tr != 4 && (n2 == 1024 || n2 == 1280 || n2 == 2048 || n2 == 2560) && n1 % 4 == 0)
@@ -272,6 +274,10 @@ static int plan_fft(soi_plan_t *pl)
       if (cv == 2 && pl->n1 == 2560) {
         pl->clay = 2;
         pl->Tc = 4;
+      }
+      if (cv == 3 && pl->n1 == 2560 && pl->f32) {
+        pl->clay = 3;
+        pl->Tc = 8;
       }
       int lb = 0;
       while ((1L << lb) * (1L << lb) < Mh) ++lb;

@@ -385,7 +385,8 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 // (64-byte runs, two blocks per CU).
 // Tile width T: 8 (128-byte runs) or 4 (64-byte runs).  variant selects a tuning alternative
 // for some lengths: 1 = one padded LDS run per sequence (2048, 2560 at T = 8); 2 = the whole
-// 256-VGPR budget for one 4-wide block per CU (2560 at T = 4).
+// 256-VGPR budget for one 4-wide block per CU (2560 at T = 4); 3 = radices 4.4.4.4.10 (2560 at
+// T = 8, complex64 only).
 template <typename C>
 static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int lb1,
                     long Mh, int S, int T, int variant, hipStream_t st)
@@ -402,6 +403,9 @@ static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, i
   COLS(0, 2048, 16, 4, 1, 2, QFast, 16, 16, 8) COLS(0, 2560, 20, 4, 2, 2, QFast, 10, 4, 4, 4, 4)
   COLS(1, 2048, 16, 8, 1, 1, SeqMajor, 16, 16, 8) COLS(1, 2560, 20, 8, 2, 1, SeqMajor, 10, 4, 4, 4, 4)
   COLS(2, 2560, 20, 4, 1, 1, QFast, 10, 4, 4, 4, 4)
+  if constexpr (sizeof(C) == 8) {  // complex64: the radix-10 stage last needs fewer registers
+    COLS(3, 2560, 20, 8, 2, 1, QFast, 4, 4, 4, 4, 10)
+  }
 #undef COLS
   return (int)hipErrorInvalidValue;
 }

@@ -96,3 +96,26 @@ def test_fp32_full_size_vs_rocfft(soi, torch_cuda):
     err = (torch.linalg.norm(x32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
     print("N=2^27 fp32 rel-L2 vs rocFFT fp64", err, flush=True)
     assert err < FP32_TOL
+
+
+def test_fp32_fft_variants_agree(soi, orc, torch_cuda):
+    """Every FFT variant of an FP32 plan (M_hat = 2560 x 2048 at N = 2^24, P = 4) stays within
+    the FP32 bound of the FP64 oracle."""
+    torch = torch_cuda
+    N, P = 2 ** 24, 4
+    x32 = orc.uniform_input(N, 12).astype(np.complex64)
+    ref, _ = orc.compute_soi(orc.params(N, P), x32.astype(np.complex128))
+    d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    try:
+        nv = soi.fft_variants(d)
+        assert nv >= 2
+        for v in range(nv):
+            soi.set_fft_variant(d, v)
+            xt = torch.from_numpy(x32.copy()).cuda()
+            torch.cuda.synchronize()
+            soi.compute_soi(d, xt.data_ptr())
+            err = rel(xt.cpu().numpy().astype(np.complex128), ref)
+            print(f"  variant {v}: rel-L2 {err:.3e}", flush=True)
+            assert err < FP32_TOL
+    finally:
+        soi.free_soi_descriptor(d)

@@ -19,6 +19,7 @@ ARCH = os.environ.get("SOI_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["soi_kernels.hip", "soi_filter.hip", "soi_fft.hip"]
 C_SOURCES = ["pfft.c", "compress_host.c"]
+DRIVER = os.path.join(HERE, "soi_test")   # command-line driver (soi_test.c), links the library
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
 HIPCC_FLAGS = [
@@ -79,8 +80,13 @@ def build(verbose=False, incremental=False):
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs +
          [f"-L{ROCM}/lib", "-lamdhip64", "-lm"])
     os.replace(tmp, LIB)
+    _run([cc, "-O2", "-std=gnu11", "-Wall", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
+          f"-I{ROCM}/include", os.path.join(CSRC, "soi_test.c"), "-o", DRIVER + ".tmp",
+          f"-L{HERE}", "-lsoi_hip", "-Wl,-rpath,$ORIGIN", f"-L{ROCM}/lib", "-lrocfft",
+          "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lm"])
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
 
 

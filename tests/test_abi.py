@@ -162,3 +162,17 @@ def test_batch_rejects_bad_arguments():
     assert not L.soi_batch_create(arr, bufs, 0, 0)
     assert L.soi_last_status() == 1
     assert L.soi_fft_variants(ctypes.byref(d)) == 0 and L.soi_last_status() == 6
+
+
+def test_driver_rejects_bad_arguments():
+    """The command-line driver checks its arguments before touching the GPU (test.c:130)."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(soi.LIB_PATH), "soi_test")
+    assert os.path.exists(drv)
+    r = subprocess.run([drv, "--P", "8", "--input_min", "2", "--input_max", "2", "2^20"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "input kind 2" in r.stderr
+    r = subprocess.run([drv, "--P", "8", "1000"], capture_output=True, text=True)
+    assert r.returncode == 1 and "must divide N" in r.stderr
+    r = subprocess.run([drv], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr

@@ -90,6 +90,11 @@ typedef struct {
   /* register-FFT alternatives (column tile, column variant, row tile) and the one in use */
   int nvar, var;
   int vtab[16][3];
+  /* pass 1 writes 8-row tiles into tbuf (S*M_hat values) and pass 2 reads them (register
+   * path).  A buffer of the plan's own: a caller's gamma_tilde may be sized as the
+   * reference's per-rank work buffer. */
+  int tiled;
+  void *tbuf;
   soi_c64_t *tw1, *tw2, *thi, *tlo;
   /* buffers */
   soi_c64_t *io;        /* staging for host-pointer inputs */
@@ -283,6 +288,12 @@ static int plan_fft(soi_plan_t *pl)
       while ((1L << lb) * (1L << lb) < Mh) ++lb;
       pl->B1 = 1 << lb;
       pl->lb1 = lb;
+      /* tiled pass-1 output: the default for complex64 (measured faster there; for
+       * complex128 the tiled row pass spills and the split stores are slower), SOI_FFT_TILED
+       * overrides */
+      const char *et = getenv("SOI_FFT_TILED");
+      pl->tiled = (et ? atoi(et) != 0 : pl->f32) && pl->n1 % 8 == 0 &&
+                  soik_fft_rows_tiled_supported(pl->n2, pl->f32);
       fft_variants(pl);
       return 0;
     }
@@ -345,6 +356,7 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->thi);
   (void)hipFree(pl->tlo);
   (void)hipFree(pl->io);
+  (void)hipFree(pl->tbuf);
   (void)hipFree(pl->d_exps);
   (void)hipFree(pl->d_emi);
   (void)hipFree(pl->d_nwords);
@@ -458,6 +470,15 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
         goto fail_quiet;
       }
       pl->owns_at = 1;
+    }
+    if (pl->tiled) {
+      hipError_t e = hipMalloc(&pl->tbuf, (size_t)S * pl->Mh * esz);
+      if (e != hipSuccess) {
+        pl->tbuf = NULL;
+        set_err(SOI_ERR_ALLOC, "Failed to allocate the FFT tile buffer (%zu bytes): %s",
+                (size_t)S * pl->Mh * esz, hipGetErrorString(e));
+        goto fail_quiet;
+      }
     }
     KCHK(soik_make_tables(pl->taps, pl->post, (soi_c64_t *)d->w, (soi_c64_t *)d->W_inv, (int)S,
                           d->n_mu, d->d_mu, (int)d->B, B_pad, d->tau, d->sigma, pl->N, M, pl->Mh,
@@ -675,11 +696,16 @@ static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, hipStream_t 
                              pl->single_count, st));
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
+    soi_c32_t *gt = pl->tiled ? (soi_c32_t *)pl->tbuf : NULL;
     KCHK(soik_fft_cols_reg_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->lb1,
-                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, st));
+                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, gt, st));
     mark(pl, 3, st);
-    KCHK(soik_fft_rows_reg_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
-                               (int)pl->S, pl->Tr, st));
+    if (gt)
+      KCHK(soik_fft_rows_tiled_f32(gt, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
+                                   (int)pl->S, st));
+    else
+      KCHK(soik_fft_rows_reg_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
+                                 (int)pl->S, pl->Tr, st));
   } else {
     KCHK(soik_fft_col_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->Tc, pl->B1,
                           pl->Mh, (int)pl->S, st));
@@ -703,11 +729,16 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, hipStream_t st)
                          pl->single_count, st));
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
+    soi_c64_t *gt = pl->tiled ? (soi_c64_t *)pl->tbuf : NULL;
     KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
-                           (int)pl->S, pl->Tc, pl->clay, st));
+                           (int)pl->S, pl->Tc, pl->clay, gt, st));
     mark(pl, 3, st);
-    KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Mh,
-                           pl->M, (int)pl->S, pl->Tr, st));
+    if (gt)
+      KCHK(soik_fft_rows_tiled(gt, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
+                               pl->Mh, pl->M, (int)pl->S, st));
+    else
+      KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
+                             pl->Mh, pl->M, (int)pl->S, pl->Tr, st));
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
                       (int)pl->S, st));

@@ -214,12 +214,13 @@ __device__ __forceinline__ int out_index(int ts, int c, int k)
 // waves per SIMD the register budget is sized for, with BPC blocks resident per CU
 constexpr int fft_minw(int BPC, int threads) { return (BPC * threads / 64 + 3) / 4; }
 
-template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C>
+template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C, bool TILED>
 __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft_cols(C *__restrict__ data,
                                                           const C *__restrict__ tw,
                                                           const C *__restrict__ thi,
                                                           const C *__restrict__ tlo,
-                                                          int n2, int lb1, long Mh, int S)
+                                                          int n2, int lb1, long Mh, int S,
+                                                          C *__restrict__ tout)
 {
   constexpr int TS = L / E;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -251,7 +252,57 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
       const int m1 = out_index<L, TS, RS>(ts, c, k);
       const unsigned ex = (unsigned)(c0 + q) * (unsigned)m1;   // < M_hat < 2^32
       const C t = cmul(thi[ex >> lb1], tlo[ex & ((1u << lb1) - 1)]);
-      seg[(unsigned)m1 * (unsigned)n2s] = cmul(v[c * RL + k], t);
+      if constexpr (TILED)  // 8-row tiles: element (m1, r2) at ((m1/8)*n2 + r2)*8 + m1%8
+        tout[(long)s * Mh + (long)(((unsigned)(m1 >> 3) * (unsigned)n2 + (unsigned)(c0 + q)) * 8u +
+                                   (unsigned)(m1 & 7))] = cmul(v[c * RL + k], t);
+      else
+        seg[(unsigned)m1 * (unsigned)n2s] = cmul(v[c * RL + k], t);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pass 2 over the 8-row tiles written by pass 1 (TILED): the block's 8 rows are one contiguous
+// run of 8*L values.  Lanes run over the 8 rows first, so the loads are contiguous and the
+// demodulated stores X[a + q + n1*m2] leave in 128-byte (64-byte for complex64) runs with no
+// transpose; exchanges use the sequence-index-fastest LDS layout.  Row blocks of one segment
+// are adjacent in launch order, so the stores of concurrently running blocks are adjacent too.
+// ---------------------------------------------------------------------------------------------
+template <int L, int E, int BPC, typename RS, typename C>
+__global__ __launch_bounds__(8 * (L / E), fft_minw(BPC, 8 * (L / E))) void k_fft_rows_t(const C *__restrict__ data,
+                                                          C *__restrict__ X,
+                                                          const C *__restrict__ tw,
+                                                          const double2 *__restrict__ winv,
+                                                          int n1, long Mh, long M, int S)
+{
+  constexpr int T = 8, TS = L / E;
+  using LY = QFast<L, T>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
+  const int tid = threadIdx.x;
+  const int q = tid % T, ts = tid / T;
+  const int nrb = n1 / T;
+  const int rb = blockIdx.x % nrb, s = blockIdx.x / nrb;
+  if (s >= S) return;
+  const int a = rb * T;
+  const C *tile = data + (long)s * Mh + (long)a * L + q;
+
+  C v[E];
+  constexpr int R0 = RS::at(0);
+#pragma unroll
+  for (int c = 0; c < E / R0; ++c)
+#pragma unroll
+    for (int k = 0; k < R0; ++k) v[c * R0 + k] = tile[(unsigned)(ts + TS * c + k * (L / R0)) * 8u];
+
+  run_stages<L, E, T, 1, TS, RS, LY, 0, C>(v, ldsd, q, ts, tw);
+
+  constexpr int RL = RS::at(RS::n - 1);
+  C *out = X + (long)s * M;
+#pragma unroll
+  for (int c = 0; c < E / RL; ++c)
+#pragma unroll
+    for (int k = 0; k < RL; ++k) {
+      const long m = (long)(a + q) + (long)n1 * out_index<L, TS, RS>(ts, c, k);
+      if (m < M) out[m] = demod<C>(winv[m], v[c * RL + k]);  // W_inv applied in FP64
     }
 }
 
@@ -332,9 +383,21 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 // ---------------------------------------------------------------------------------------------
 template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C>
 static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
-                       int n2, int lb1, long Mh, int S, hipStream_t st)
+                       int n2, int lb1, long Mh, int S, C *tout, hipStream_t st)
 {
-  auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C>;
+  if (tout) {
+    auto kt = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, true>;
+    constexpr size_t lds = (size_t)LY::size * sizeof(real_t<C>);
+    static bool once_t = false;
+    if (!once_t) {
+      (void)hipFuncSetAttribute((const void *)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      once_t = true;
+    }
+    hipLaunchKernelGGL(kt, dim3((unsigned)((n2 / T) * S)), dim3(T * (L / E)), lds, st, data, tw,
+                       thi, tlo, n2, lb1, Mh, S, tout);
+    return (int)hipGetLastError();
+  }
+  auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, false>;
   constexpr size_t lds = (size_t)LY::size * sizeof(real_t<C>);
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
@@ -343,7 +406,7 @@ static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
     once = true;
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)((n2 / T) * S)), dim3(T * (L / E)), lds, st, data, tw,
-                     thi, tlo, n2, lb1, Mh, S);
+                     thi, tlo, n2, lb1, Mh, S, (C *)nullptr);
   return (int)hipGetLastError();
 }
 
@@ -360,6 +423,23 @@ static int launch_rows(const C *data, C *X, const C *tw, const double2 *winv,
     once = true;
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / T) * S)), dim3(T * (L / E)), lds, st, data, X,
+                     tw, winv, n1, Mh, M, S);
+  return (int)hipGetLastError();
+}
+
+template <int L, int E, int BPC, typename RS, typename C>
+static int launch_rows_t(const C *data, C *X, const C *tw, const double2 *winv, int n1, long Mh,
+                         long M, int S, hipStream_t st)
+{
+  auto kern = k_fft_rows_t<L, E, BPC, RS, C>;
+  constexpr size_t lds = (size_t)QFast<L, 8>::size * sizeof(real_t<C>);
+  static_assert(lds <= 160 * 1024, "LDS");
+  static bool once = false;
+  if (!once) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    once = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / 8) * S)), dim3(8 * (L / E)), lds, st, data, X,
                      tw, winv, n1, Mh, M, S);
   return (int)hipGetLastError();
 }
@@ -389,12 +469,12 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 // T = 8, complex64 only).
 template <typename C>
 static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int lb1,
-                    long Mh, int S, int T, int variant, hipStream_t st)
+                    long Mh, int S, int T, int variant, C *tout, hipStream_t st)
 {
   if (n2 % T) return (int)hipErrorInvalidValue;
 #define COLS(VV, LV, EV, TV, NQ, BPC, LY, ...) \
   if (n1 == LV && T == TV && variant == VV) \
-    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>, C>(data, tw1, thi, tlo, n2, lb1, Mh, S, st);
+    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>, C>(data, tw1, thi, tlo, n2, lb1, Mh, S, tout, st);
   COLS(0, 256, 16, 8, 1, 1, QFast, 16, 16) COLS(0, 320, 20, 8, 1, 1, QFast, 10, 4, 4, 2)
   COLS(0, 512, 16, 8, 1, 1, QFast, 16, 16, 2) COLS(0, 640, 20, 8, 1, 1, QFast, 10, 4, 4, 4)
   COLS(0, 1024, 16, 8, 1, 1, QFast, 16, 16, 4) COLS(0, 1280, 20, 8, 1, 1, QFast, 10, 4, 4, 4, 2)
@@ -427,11 +507,49 @@ static int rows_reg(const C *data, C *X, const C *tw2, const double2 *winv, int 
   return (int)hipErrorInvalidValue;
 }
 
+template <typename C>
+static int rows_t(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2, long Mh,
+                  long M, int S, hipStream_t st)
+{
+  if (n1 % 8) return (int)hipErrorInvalidValue;
+#define ROWST(LV, EV, BPC, ...) \
+  if (n2 == LV) return launch_rows_t<LV, EV, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, st);
+  ROWST(256, 16, 1, 16, 16) ROWST(320, 20, 1, 10, 4, 4, 2) ROWST(512, 16, 1, 16, 16, 2)
+  ROWST(640, 20, 1, 10, 4, 4, 4) ROWST(1024, 16, 1, 16, 16, 4)
+  ROWST(1280, 20, 1, 10, 4, 4, 4, 2) ROWST(2048, 16, 1, 16, 16, 8)
+  if constexpr (sizeof(C) == 8) { ROWST(2560, 20, 1, 10, 4, 4, 4, 4) }
+#undef ROWST
+  return (int)hipErrorInvalidValue;
+}
+
+extern "C" int soik_fft_rows_tiled_supported(int n2, int f32)
+{
+  switch (n2) {
+    case 256: case 320: case 512: case 640: case 1024: case 1280: case 2048: return 1;
+    case 2560: return f32;
+    default: return 0;
+  }
+}
+
 extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
                                  const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                 int T, int variant, hipStream_t st)
+                                 int T, int variant, double2 *tout, hipStream_t st)
 {
-  return cols_reg<double2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, st);
+  return cols_reg<double2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, st);
+}
+
+extern "C" int soik_fft_rows_tiled(const double2 *data, double2 *X, const double2 *tw2,
+                                   const double2 *winv, int n1, int n2, long Mh, long M, int S,
+                                   hipStream_t st)
+{
+  return rows_t<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, st);
+}
+
+extern "C" int soik_fft_rows_tiled_f32(const float2 *data, float2 *X, const float2 *tw2,
+                                       const double2 *winv, int n1, int n2, long Mh, long M,
+                                       int S, hipStream_t st)
+{
+  return rows_t<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, st);
 }
 
 extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 *tw2,
@@ -444,9 +562,9 @@ extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 
 // FP32 form: complex64 data and tables (rounded from their FP64 values); W_inv stays FP64.
 extern "C" int soik_fft_cols_reg_f32(float2 *data, const float2 *tw1, const float2 *thi,
                                      const float2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                     int T, int variant, hipStream_t st)
+                                     int T, int variant, float2 *tout, hipStream_t st)
 {
-  return cols_reg<float2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, st);
+  return cols_reg<float2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, st);
 }
 
 extern "C" int soik_fft_rows_reg_f32(const float2 *data, float2 *X, const float2 *tw2,

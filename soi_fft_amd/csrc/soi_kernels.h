@@ -56,13 +56,21 @@ int soik_fft_row_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
 int soik_fft_reg_supported(int L, int rows);
 int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                       const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
-                      int layout, hipStream_t st);
+                      int layout, soi_c64_t *tout, hipStream_t st);
 int soik_fft_rows_reg(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                       const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
                       hipStream_t st);
 int soik_fft_cols_reg_f32(soi_c32_t *data, const soi_c32_t *tw1, const soi_c32_t *thi,
                           const soi_c32_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
-                          int layout, hipStream_t st);
+                          int layout, soi_c32_t *tout, hipStream_t st);
+/* pass 2 over pass 1's 8-row tiles (tout of the column pass) */
+int soik_fft_rows_tiled_supported(int n2, int f32);
+int soik_fft_rows_tiled(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
+                        const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S,
+                        hipStream_t st);
+int soik_fft_rows_tiled_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
+                            const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S,
+                            hipStream_t st);
 int soik_fft_rows_reg_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
                           const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
                           hipStream_t st);

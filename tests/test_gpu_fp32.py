@@ -98,10 +98,12 @@ def test_fp32_full_size_vs_rocfft(soi, torch_cuda):
     assert err < FP32_TOL
 
 
-def test_fp32_fft_variants_agree(soi, orc, torch_cuda):
+@pytest.mark.parametrize("tiled", ["0", "1"])
+def test_fp32_fft_variants_agree(soi, orc, torch_cuda, monkeypatch, tiled):
     """Every FFT variant of an FP32 plan (M_hat = 2560 x 2048 at N = 2^24, P = 4) stays within
     the FP32 bound of the FP64 oracle."""
     torch = torch_cuda
+    monkeypatch.setenv("SOI_FFT_TILED", tiled)
     N, P = 2 ** 24, 4
     x32 = orc.uniform_input(N, 12).astype(np.complex64)
     ref, _ = orc.compute_soi(orc.params(N, P), x32.astype(np.complex128))

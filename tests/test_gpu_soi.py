@@ -288,6 +288,8 @@ FFT_VARIANTS = [
     (2 ** 24, 4, {"SOI_FFT_CVAR": "0"}),
     (2 ** 24, 4, {"SOI_FFT_N1": "2048"}),
     (2 ** 24, 4, {"SOI_FFT_N1": "2048", "SOI_FFT_TC": "4"}),
+    (2 ** 24, 4, {"SOI_FFT_TILED": "1"}),
+    (2 ** 24, 8, {"SOI_FFT_TILED": "1"}),
 ]
 
 
@@ -302,7 +304,7 @@ def test_fft_kernel_variants(soi, torch_cuda, monkeypatch, N, P, env):
     ref = torch.fft.fft(xt)
     outs = []
     for e in ({}, env):
-        for kk in ("SOI_FFT_TC", "SOI_FFT_TR", "SOI_FFT_CVAR", "SOI_FFT_N1"):
+        for kk in ("SOI_FFT_TC", "SOI_FFT_TR", "SOI_FFT_CVAR", "SOI_FFT_N1", "SOI_FFT_TILED"):
             monkeypatch.delenv(kk, raising=False)
         for kk, v in e.items():
             monkeypatch.setenv(kk, v)

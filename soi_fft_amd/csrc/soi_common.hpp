@@ -219,6 +219,15 @@ __device__ __forceinline__ void lds_fft(C *buf, int L, int count, int qs, int es
   }
 }
 
+// Barrier for LDS hand-off only: waits for this wave's LDS operations, not for its global loads
+// and stores, which stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // XCD-aware block index: hardware dispatches consecutive workgroups round-robin over the 8 XCDs.
 // Remapping makes logically adjacent blocks (which share table rows and cache lines) land on the
 // same XCD and its L2.

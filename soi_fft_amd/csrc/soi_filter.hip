@@ -86,20 +86,16 @@ struct FilterCfg {
   // blocks resident per CU (LDS and registers), and the register budget that follows
   static constexpr int BPC = (LDS_BYTES <= 80 * 1024 && T <= 320) ? 2 : 1;
   static constexpr int MINW = (BPC * NWAVES + 3) / 4;
+  // two real-tap slices whose partial sums meet by a lane exchange inside the wave: the slice
+  // index is lane bit log2(S)
+  // (complex64 only: for complex128 the lanes of the two phases that read the same rows are
+  // worth more to the FMA block's LDS traffic, measured)
+  static constexpr bool LSW = !CT && NP == 2 && S <= 32 && U % 2 == 0 && sizeof(C) == 8;
   static constexpr int NSC = (GR * S + T - 1) / T;   // stores per thread per tile (at most)
   static constexpr int NSC_MIN = (GR * S) / T;       // (at least)
 };
 
 constexpr int filter_threads(int S, int NT, int NP, int W) { return S * NT * NP * W; }
-
-// Barrier for LDS hand-off only: waits for this wave's LDS operations, not for its global
-// stores, which stay in flight across it.
-__device__ __forceinline__ void lds_barrier()
-{
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
 
 // s_waitcnt vmcnt(n) (expcnt and lgkmcnt left alone), through the builtin so the compiler's
 // own wait tracking sees it
@@ -165,6 +161,26 @@ __device__ __forceinline__ void sdft_passes(C *G, const C *tw, int tid)
   }
 }
 
+// Exchange of the two tap-slice partial sums held by lanes l and l ^ S (slice p = lane bit
+// log2 S), complex64: a = acc[u], b = acc[u + U/2] on entry; on return the slice-0 lane holds
+// acc0[u] + acc1[u] and the slice-1 lane acc0[u + U/2] + acc1[u + U/2].
+template <int S>
+__device__ __forceinline__ float2 slice_sum(float2 a, float2 b, int p)
+{
+  if constexpr (S == 32) {
+    // v_permlane32_swap: lanes 32..63 of the first operand trade with lanes 0..31 of the second
+    const auto re = __builtin_amdgcn_permlane32_swap(__float_as_uint(a.x), __float_as_uint(b.x),
+                                                     false, false);
+    const auto im = __builtin_amdgcn_permlane32_swap(__float_as_uint(a.y), __float_as_uint(b.y),
+                                                     false, false);
+    return float2{__uint_as_float(re[0]) + __uint_as_float(re[1]),
+                  __uint_as_float(im[0]) + __uint_as_float(im[1])};
+  } else {
+    const float2 keep = p ? b : a, send = p ? a : b;
+    return cadd(keep, float2{__shfl_xor(send.x, S), __shfl_xor(send.y, S)});
+  }
+}
+
 // Stage the XN input values of a tile (contiguous from element g0, wrapping once at N) into
 // dst with direct-to-LDS loads; wave w moves 64-value chunks w, w + NWAVES, ...
 template <typename CF, typename C>
@@ -216,9 +232,18 @@ __device__ __forceinline__ void filter_tile(const FilterThread<BP, CT, C> &th, C
                                                       th.h, acc);
   lds_barrier();
 
-  // ---- tap-slice reduction (fixed order p = NP-1 .. 0) and post phase, into G = cur
+  // ---- tap-slice reduction and post phase, into G = cur
   C *G = cur;
   const int gbase = (th.w * U) * NT + th.tl;
+  if constexpr (CF::LSW) {
+    // lane exchange: each slice finishes half of the U outputs (acc0 + acc1 in both)
+#pragma unroll
+    for (int u = 0; u < U / 2; ++u) {
+      const C a = cmul(slice_sum<S>(acc[u], acc[u + U / 2], th.p), th.pf);
+      G[(gbase + (u + th.p * (U / 2)) * NT) * CF::GS + th.i] = a;
+    }
+    lds_barrier();
+  } else
 #pragma unroll
   for (int q = NP - 1; q >= 0; --q) {
     if (th.p == q) {
@@ -277,10 +302,17 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
   th.lane = th.tid & 63;
   th.i = th.tid % S;
   int rest = th.tid / S;
-  th.tl = rest % NT;
-  rest /= NT;
-  th.p = rest % NP;
-  th.w = rest / NP;
+  if constexpr (CF::LSW) {  // (i, p, theta, w), p a lane bit
+    th.p = rest % NP;
+    rest /= NP;
+    th.tl = rest % NT;
+    th.w = rest / NT;
+  } else {                  // (i, theta, p, w)
+    th.tl = rest % NT;
+    rest /= NT;
+    th.p = rest % NP;
+    th.w = rest / NP;
+  }
 
   // phase group: the slow index of the block id so blocks sharing input rows are 8-apart
   // (same XCD under round-robin dispatch) and walk the same tiles at the same time

@@ -84,6 +84,7 @@ typedef struct {
   /* FP32 form: copies of the FP64 tables rounded to FP32 (W_inv stays FP64) */
   int f32;
   soi_c32_t *taps32, *post32, *twS32, *tw1_32, *tw2_32, *thi32, *tlo32;
+  void *wt; /* complex64 MFMA filter: bf16 tap pieces */
   /* fft */
   int fft_single, single_count, fft_reg;
   int n1, n2, Tc, Tr, B1, lb1, clay;
@@ -345,6 +346,7 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->post);
   (void)hipFree(pl->twS);
   (void)hipFree(pl->taps32);
+  (void)hipFree(pl->wt);
   (void)hipFree(pl->post32);
   (void)hipFree(pl->twS32);
   (void)hipFree(pl->tw1_32);
@@ -514,6 +516,16 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       pl->fa.post = pl->post32;
       pl->fa.twS = pl->twS32;
       pl->fa.f32 = 1;
+      /* SOI_FILTER_MFMA=1: the filter contraction on MFMA with split-bf16 operands where its
+       * shape allows.  Off by default: at S = 32 the per-lane tap operands take 144 of the
+       * 256 registers of its 512-thread blocks and it runs slower than the FP32 VALU form. */
+      const char *em = getenv("SOI_FILTER_MFMA");
+      if (em && atoi(em) != 0 &&
+          soik_filter_mfma_supported((int)S, d->n_mu, d->d_mu, pl->fa.ct, B_pad)) {
+        HIPCHK(hipMalloc(&pl->wt, soik_filter_mfma_table_bytes((int)S, B_pad)));
+        KCHK(soik_filter_mfma_table((const double *)pl->taps, pl->wt, (int)S, B_pad, pl->stream));
+        pl->fa.wt = pl->wt;
+      }
       if (pl->fft_single) {
         HIPCHK(hipMalloc((void **)&pl->tw1_32, (size_t)pl->Mh * sizeof(soi_c32_t)));
         KCHK(soik_f64_to_f32(pl->tw1, pl->tw1_32, pl->Mh, pl->stream));

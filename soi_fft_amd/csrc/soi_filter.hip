@@ -362,6 +362,267 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
 }
 
 // ---------------------------------------------------------------------------------------------
+// Complex64 real-tap filter on MFMA with split-bf16 operands: every FP32 value is hi + mid + lo
+// bf16 pieces (24 significant bits), and six of the nine piece products (hh, hm, mh, hl, lh, mm;
+// the dropped ones are below 2^-24 relative) accumulate in FP32.
+//
+// For lane i, the outputs of row groups J = J0 + 3m + D (m < 8, D < 3) and phase theta form one
+// 16 x 16 x K real product per tile and lane:
+//   C[2m + part][theta + 5D] = sum_k x_i[4 J0 + 12 m + k].part * t_i[theta][k - 4D]
+// with t_i[theta][b] the real taps (zero outside [0, BPAD)) and K = BPAD + 8 <= 96 covered by
+// three 32-deep chunks (v_mfma_f32_16x16x32_bf16; every accumulation chain stays on this one
+// opcode).
+// So R = 5 J + theta = 5 J0 + 15 m + n for column n = theta + 5D < 15.  A wave owns LPW lanes i
+// and keeps their B operands (the tap pieces) in registers for the whole kernel; the A operands
+// come from an LDS copy of the tile's input rows transposed to one run per (lane, piece, part).
+//
+// Per tile: the raw rows of the next tile arrive by direct-to-LDS loads while this one computes;
+// the split pass turns raw rows into bf16 pieces; MFMA; post-phase multiply into G; the S-point
+// DFT of each row of G; coalesced stores of 120 consecutive R per segment.
+// ---------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int S> struct MfmaCfg {
+  static constexpr int EPL = 2;                 // complex64 values per 16-byte direct-to-LDS load
+  static constexpr int TJ = 24;                 // row groups per tile: 8 MFMA rows x 3 shifts
+  static constexpr int NCH = 3;                 // 32-deep K chunks
+  static constexpr int K = 32 * NCH;
+  static constexpr int NRG = (12 * 7 + K + 7) / 8;  // 8-row groups staged per tile
+  static constexpr int ROWS = NRG * 8;
+  static constexpr int RP = ROWS;               // bf16 pitch of one (lane, piece, part) run
+  static constexpr int LPW = S >= 32 ? 4 : S < 2 ? S : 2;  // lanes per wave (<= 8 waves)
+  static constexpr int NWAVES = S / LPW;
+  static constexpr int T = NWAVES * 64;
+  static constexpr int XN = (ROWS * S + 64 * EPL - 1) / (64 * EPL) * (64 * EPL);  // raw values
+  static constexpr int NDMA = XN / (64 * EPL);  // wave-wide 16-byte loads per tile
+  static constexpr int GR = TJ * 5;             // output rows R per tile
+  static constexpr int GS = S + 1;
+  static constexpr int NSC = (GR * S + T - 1) / T;   // stores per thread per tile (at most)
+  static constexpr int NSC_MIN = (GR * S) / T;        // (at least)
+};
+
+// hi + mid + lo bf16 split of an FP32 value (each residual is exact in FP32)
+__device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l)
+{
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+template <int S>
+__global__ __launch_bounds__(MfmaCfg<S>::T, MfmaCfg<S>::T / 256) void k_filter_mfma(
+    const float2 *__restrict__ x, float2 *__restrict__ at, const bf16x8 *__restrict__ wt,
+    const float2 *__restrict__ post, const float2 *__restrict__ twS, long M, long Mh, long nJ,
+    int tiles_per_block)
+{
+  typedef MfmaCfg<S> CF;
+  constexpr int LPW = CF::LPW, NCH = CF::NCH, RP = CF::RP;
+  __shared__ __align__(16) float2 raw[CF::XN];
+  __shared__ __align__(16) __bf16 As[S * 6 * RP];
+  __shared__ __align__(16) float2 G[CF::GR * CF::GS];
+  __shared__ __align__(16) float2 tw[S];
+  __shared__ __align__(16) float2 pfl[5 * S];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int n = lane & 15, g = lane >> 4;
+
+  // B operands (tap pieces) of this wave's lanes
+  bf16x8 wb[LPW][NCH][3];
+#pragma unroll
+  for (int li = 0; li < LPW; ++li) {
+    const int i = wave * LPW + li;
+#pragma unroll
+    for (int kc = 0; kc < NCH; ++kc)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) wb[li][kc][p] = wt[((i * NCH + kc) * 3 + p) * 64 + lane];
+  }
+  for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
+  for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
+
+  const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
+  const long t0 = (long)blockIdx.x * tiles_per_block;
+  const long t1 = min(ntiles, t0 + tiles_per_block);
+  const long N = M * S;
+  if (t0 >= t1) return;
+
+  // raw rows of a tile -> bf16 pieces, As[((i*3 + p)*2 + part)*RP + row]
+  auto split_tile = [&]() {
+    for (int task = tid; task < CF::NRG * S; task += CF::T) {
+      const int i = task % S, rg = task / S;
+#pragma unroll
+      for (int part = 0; part < 2; ++part) {
+        bf16x8 pc[3];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const float2 v = raw[(rg * 8 + r) * S + i];
+          __bf16 h, m, l;
+          split3(part ? v.y : v.x, h, m, l);
+          pc[0][r] = h, pc[1][r] = m, pc[2][r] = l;
+        }
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+          *reinterpret_cast<bf16x8 *>(As + ((i * 3 + p) * 2 + part) * RP + rg * 8) = pc[p];
+      }
+    }
+  };
+
+  // the raw rows of tile t + 1 stream in from the end of tile t - 1's pieces split to the end
+  // of tile t's MFMA block
+  dma_rows<CF, float2>(raw, x, t0 * CF::TJ * 4 * S, N, wave, lane);
+  wait_vmcnt<0>();
+  lds_barrier();
+  split_tile();
+  lds_barrier();
+  if (t0 + 1 < t1) dma_rows<CF, float2>(raw, x, (t0 + 1) * CF::TJ * 4 * S, N, wave, lane);
+
+  // A operand row of this lane: (m, part) = (r >> 1, r & 1), r = lane % 16
+  const int am = n >> 1, apart = n & 1;
+  for (long tile = t0; tile < t1; ++tile) {
+    const bool more = tile + 1 < t1;
+
+    // ---- MFMA: C_li = sum over K chunks of the six piece products
+    f32x4 acc[LPW];
+#pragma unroll
+    for (int li = 0; li < LPW; ++li) acc[li] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int kc = 0; kc < NCH; ++kc) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        const int i = wave * LPW + li;
+        bf16x8 a[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {  // K offset 8g: 8 rows at an 8-byte aligned address
+          const __bf16 *src = As + ((i * 3 + p) * 2 + apart) * RP + 12 * am + 32 * kc + 8 * g;
+          const bf16x4 lo4 = *reinterpret_cast<const bf16x4 *>(src);
+          const bf16x4 hi4 = *reinterpret_cast<const bf16x4 *>(src + 4);
+          a[p] = bf16x8{lo4[0], lo4[1], lo4[2], lo4[3], hi4[0], hi4[1], hi4[2], hi4[3]};
+        }
+        f32x4 c = acc[li];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], wb[li][kc][1], c, 0, 0, 0);  // mm
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], wb[li][kc][0], c, 0, 0, 0);  // lh
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], wb[li][kc][2], c, 0, 0, 0);  // hl
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], wb[li][kc][0], c, 0, 0, 0);  // mh
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], wb[li][kc][1], c, 0, 0, 0);  // hm
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], wb[li][kc][0], c, 0, 0, 0);  // hh
+        acc[li] = c;
+      }
+    }
+    lds_barrier();  // every wave is done reading As
+
+    // ---- post phase into G: accumulator j of this lane is part j & 1 of row m = 2g + (j >> 1)
+    if (n < 15) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        const int i = wave * LPW + li;
+        const float2 pf = pfl[(n % 5) * S + i];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          G[(15 * (2 * g + h) + n) * CF::GS + i] = cmul(float2{acc[li][2 * h], acc[li][2 * h + 1]}, pf);
+      }
+    }
+    if (more) {
+      // the next tile's raw rows are complete once at most this wave's stores of the previous
+      // tile are outstanding (they were issued after the rows; none before the first tile)
+      asm volatile("" ::: "memory");
+      if (tile == t0) wait_vmcnt<0>();
+      else wait_vmcnt<CF::NSC_MIN>();
+      lds_barrier();
+      split_tile();
+    }
+    lds_barrier();  // publishes G and the next tile's pieces; raw is free again
+    if (tile + 2 < t1) dma_rows<CF, float2>(raw, x, (tile + 2) * CF::TJ * 4 * S, N, wave, lane);
+
+    // ---- S-point DFT of each row
+    sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, float2>(G, tw, tid);
+
+    // ---- store segment-major: R = 5 J0 + r (runs of GR consecutive R per segment)
+    const long R0 = tile * CF::TJ * 5;
+#pragma unroll
+    for (int c = 0; c < CF::NSC; ++c) {
+      const int e = tid + c * CF::T;
+      if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
+        const int s = e / CF::GR, r = e % CF::GR;
+        if (tile * CF::TJ + r / 5 < nJ) store_nt(at + (long)s * Mh + R0 + r, G[r * CF::GS + s]);
+      }
+    }
+  }
+}
+
+// B operand table: wt[((i*NCH + ch)*3 + p)*64 + lane] holds bf16 piece p (hi, mid, lo) of
+// t_i[theta][k - 4D] for k = 32 ch + 8 (lane/16) + j (j < 8) and n = lane % 16 = theta + 5D
+// (zero for n = 15 or a tap index outside [0, BPAD)); t from the FP64 real-tap table
+// taps[(theta*S + i)*BPAD + b].
+__global__ void k_filter_mfma_table(const double *__restrict__ taps, unsigned short *__restrict__ wt,
+                                    int S, int BPAD, int NCH)
+{
+  const long total = (long)S * NCH * 64 * 8;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(e % 8), lane = (int)(e / 8 % 64), ch = (int)(e / 512 % NCH), i = (int)(e / (512L * NCH));
+    const int n = lane % 16;
+    const int k = 32 * ch + 8 * (lane / 16) + j;
+    const int theta = n % 5, D = n / 5, b = k - 4 * D;
+    const bool on = n < 15 && b >= 0 && b < BPAD;
+    const double t = on ? taps[((long)theta * S + i) * BPAD + b] : 0.0;
+    const __bf16 h = (__bf16)(float)t;
+    const double r1 = t - (double)(float)h;
+    const __bf16 m = (__bf16)(float)r1;
+    const __bf16 l = (__bf16)(float)(r1 - (double)(float)m);
+    const long base = (((long)i * NCH + ch) * 3) * 64 * 8 + lane * 8 + j;
+    wt[base] = __builtin_bit_cast(unsigned short, h);
+    wt[base + 512] = __builtin_bit_cast(unsigned short, m);
+    wt[base + 1024] = __builtin_bit_cast(unsigned short, l);
+  }
+}
+
+template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
+{
+  typedef MfmaCfg<S> CF;
+  if ((long)CF::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
+  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
+  long nb = 256;
+  if (nb > ntiles) nb = ntiles;
+  const int tpb = (int)((ntiles + nb - 1) / nb);
+  nb = (ntiles + tpb - 1) / tpb;
+  hipLaunchKernelGGL(k_filter_mfma<S>, dim3((unsigned)nb), dim3(CF::T), 0, st,
+                     (const float2 *)a->x, (float2 *)a->at, (const bf16x8 *)a->wt,
+                     (const float2 *)a->post, (const float2 *)a->twS, a->M, a->Mh, a->nJ, tpb);
+  return (int)hipGetLastError();
+}
+
+// The MFMA form applies to the complex64 real-tap (5, 4) filter with S in {4, 8, 16, 32} and at
+// most 88 padded taps.
+extern "C" int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD)
+{
+  return !ct && n_mu == 5 && d_mu == 4 && BPAD <= 88 && (S == 4 || S == 8 || S == 16 || S == 32);
+}
+
+extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD)
+{
+  (void)BPAD;
+  return (size_t)S * MfmaCfg<4>::NCH * 3 * 64 * 16;
+}
+
+extern "C" int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_filter_mfma_table, dim3(64), dim3(256), 0, st, taps, (unsigned short *)wt, S,
+                     BPAD, MfmaCfg<4>::NCH);
+  return (int)hipGetLastError();
+}
+
+static int filter_mfma(const SoiFilterArgs *a, hipStream_t st)
+{
+  switch (a->S) {
+    case 4: return launch_mfma<4>(a, st);
+    case 8: return launch_mfma<8>(a, st);
+    case 16: return launch_mfma<16>(a, st);
+    case 32: return launch_mfma<32>(a, st);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Dispatch table
 // ---------------------------------------------------------------------------------------------
 template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C>
@@ -469,6 +730,7 @@ extern "C" size_t soik_filter_lds_bytes(const SoiFilterArgs *a)
 
 extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
 {
+  if (a->f32 && a->wt) return filter_mfma(a, st);
   if (!a->ct) {
     RT_CASE(4, 32, 16) RT_CASE(4, 36, 16) RT_CASE(4, 40, 16)
     RT_CASE(8, 32, 8) RT_CASE(8, 36, 8) RT_CASE(8, 40, 8)

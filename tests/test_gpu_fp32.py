@@ -121,3 +121,22 @@ def test_fp32_fft_variants_agree(soi, orc, torch_cuda, monkeypatch, tiled):
             assert err < FP32_TOL
     finally:
         soi.free_soi_descriptor(d)
+
+
+MFMA_SHAPES = [(2 ** 13, 4), (2 ** 16, 8), (2 ** 18, 16), (2 ** 18, 32), (2 ** 20, 8)]
+
+
+@pytest.mark.parametrize("N,P", MFMA_SHAPES)
+def test_fp32_mfma_filter_matches_fp64_oracle(soi, orc, torch_cuda, monkeypatch, N, P):
+    """The split-bf16 MFMA filter (SOI_FILTER_MFMA=1: hi + mid + lo bf16 pieces per operand,
+    six piece products, FP32 accumulation) for S = 4 .. 32 lanes, one and several tiles per
+    block, against the FP64 oracle at the FP32 bound."""
+    monkeypatch.setenv("SOI_FILTER_MFMA", "1")
+    print(f"case N={N} P={P} mfma filter", flush=True)
+    x32 = orc.uniform_input(N, 303).astype(np.complex64)
+    got = run_soi32(soi, torch_cuda, x32, P)
+    p = orc.params(N, P)
+    ref, _ = orc.compute_soi(p, x32.astype(np.complex128))
+    err = rel(got.astype(np.complex128), ref)
+    print(f"  mfma fp32 rel-L2 vs fp64 oracle {err:.3e}", flush=True)
+    assert err < FP32_TOL

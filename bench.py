@@ -182,7 +182,8 @@ def bench_batch(args, soi, torch, prec):
             "ms_per_step": round(ms, 4), "transforms": len(shapes),
             "points": int(sum(n for n, _ in shapes)), "setup_s": round(setup_s, 2),
             "dtype": "f32" if prec else "f64",
-            "launch": "one HIP graph launch per step (4 concurrent capture branches)"}
+            "launch": "one HIP graph launch per step; inside it one launch per kernel per group of "
+                      "same-shape transforms (blockIdx.y = transform), groups over 4 capture branches"}
 
 
 def cpu_baseline(log2n, P):

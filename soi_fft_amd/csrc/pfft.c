@@ -653,11 +653,13 @@ static void vlc_plan(soi_desc_t *d, soi_plan_t *pl)
   free(pcs);
 }
 
-static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, hipStream_t st)
+static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, SoiBatch b, hipStream_t st)
 {
   pl->fa.x = x;
   pl->fa.at = d->alpha_tilde;
+  pl->fa.b = b;
   int e = soik_filter_sdft(&pl->fa, st);
+  pl->fa.b = (SoiBatch){0};
   if (e) {
     set_err(SOI_ERR_HIP, "filter kernel launch failed: %s", hipGetErrorString((hipError_t)e));
     return -1;
@@ -698,65 +700,65 @@ fail:
   return -1;
 }
 
-static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, hipStream_t st)
+static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, SoiBatch b, hipStream_t st)
 {
   soi_c32_t *at = (soi_c32_t *)d->alpha_tilde;
   const soi_c64_t *winv = (const soi_c64_t *)d->W_inv;
   if (pl->fft_single) {
     mark(pl, 2, st);
     KCHK(soik_fft_single_f32(at, X, pl->tw1_32, winv, pl->Mh, pl->M, (int)pl->S,
-                             pl->single_count, st));
+                             pl->single_count, b, st));
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
     soi_c32_t *gt = pl->tiled ? (soi_c32_t *)pl->tbuf : NULL;
     KCHK(soik_fft_cols_reg_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->lb1,
-                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, gt, st));
+                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, gt, b, st));
     mark(pl, 3, st);
     if (gt)
       KCHK(soik_fft_rows_tiled_f32(gt, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
-                                   (int)pl->S, st));
+                                   (int)pl->S, b, st));
     else
       KCHK(soik_fft_rows_reg_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
-                                 (int)pl->S, pl->Tr, st));
+                                 (int)pl->S, pl->Tr, b, st));
   } else {
     KCHK(soik_fft_col_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->Tc, pl->B1,
-                          pl->Mh, (int)pl->S, st));
+                          pl->Mh, (int)pl->S, b, st));
     mark(pl, 3, st);
     KCHK(soik_fft_row_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Tr, pl->Mh, pl->M,
-                          (int)pl->S, st));
+                          (int)pl->S, b, st));
   }
   return 0;
 fail:
   return -1;
 }
 
-static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, hipStream_t st)
+static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, SoiBatch b, hipStream_t st)
 {
-  if (pl->f32) return run_fft_f32(d, pl, (soi_c32_t *)Xv, st);
+  if (pl->f32) return run_fft_f32(d, pl, (soi_c32_t *)Xv, b, st);
   soi_c64_t *X = (soi_c64_t *)Xv;
   soi_c64_t *at = (soi_c64_t *)d->alpha_tilde;
   if (pl->fft_single) {
     mark(pl, 2, st);
     KCHK(soik_fft_single(at, X, pl->tw1, (const soi_c64_t *)d->W_inv, pl->Mh, pl->M, (int)pl->S,
-                         pl->single_count, st));
+                         pl->single_count, b, st));
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
     soi_c64_t *gt = pl->tiled ? (soi_c64_t *)pl->tbuf : NULL;
     KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
-                           (int)pl->S, pl->Tc, pl->clay, gt, st));
+                           (int)pl->S, pl->Tc, pl->clay, gt, b, st));
     mark(pl, 3, st);
     if (gt)
       KCHK(soik_fft_rows_tiled(gt, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
-                               pl->Mh, pl->M, (int)pl->S, st));
+                               pl->Mh, pl->M, (int)pl->S, b, st));
     else
       KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
-                             pl->Mh, pl->M, (int)pl->S, pl->Tr, st));
+                             pl->Mh, pl->M, (int)pl->S, pl->Tr, b, st));
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
-                      (int)pl->S, st));
+                      (int)pl->S, b, st));
     mark(pl, 3, st);
     KCHK(soik_fft_row(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Tr,
-                      pl->Mh, pl->M, (int)pl->S, st));
+                      pl->Mh, pl->M, (int)pl->S, b, st));
   }
   return 0;
 fail:
@@ -796,12 +798,12 @@ static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, 
     }
     pl->vlc_valid = 0;
     mark(pl, 0, st);
-    if (run_filter(d, pl, buf, st)) return;
+    if (run_filter(d, pl, buf, (SoiBatch){0}, st)) return;
     mark(pl, 1, st);
     if (!filter_only) {
       if (d->use_vlc && run_vlc(d, pl, st)) return;
       if (!pl->fft_single) mark(pl, 2, st);
-      if (run_fft(d, pl, buf, st)) return;
+      if (run_fft(d, pl, buf, (SoiBatch){0}, st)) return;
       mark(pl, 4, st);
     }
     if (pl->timing) {
@@ -992,16 +994,19 @@ int soi_fill_tones(void *x, long N, const double *freqs, const double *phases, i
 
 /* ---- batching form: many initialised descriptors replayed as one HIP graph ---------------- */
 
-#define SOI_BATCH_BRANCHES 4 /* independent capture branches (the device's hardware queues) */
+#define SOI_BATCH_MAX_BRANCHES 16 /* independent capture branches at most */
 
 struct soi_batch_s {
   int count;
   int device;
   hipStream_t stream;
-  hipStream_t branch[SOI_BATCH_BRANCHES];
-  hipEvent_t fork, join[SOI_BATCH_BRANCHES];
+  int nbranch;
+  hipStream_t branch[SOI_BATCH_MAX_BRANCHES];
+  hipEvent_t fork, join[SOI_BATCH_MAX_BRANCHES];
   hipGraph_t graph;
   hipGraphExec_t exec;
+  int ngroups;
+  void **tabs; /* device pointer tables of the groups (host array of ngroups entries) */
 };
 
 static int same_shape(const soi_desc_t *a, const soi_desc_t *b)
@@ -1011,17 +1016,31 @@ static int same_shape(const soi_desc_t *a, const soi_desc_t *b)
          a->sigma == b->sigma && a->window_mode == b->window_mode;
 }
 
+/* Transforms that one batched launch can take together: same shape and the same kernel choices
+ * (so that the first one's tables and launch shapes serve all of them). */
+static int same_plan(const soi_desc_t *a, const soi_desc_t *b)
+{
+  const soi_plan_t *pa = (const soi_plan_t *)a->plan, *pb = (const soi_plan_t *)b->plan;
+  return same_shape(a, b) && pa->var == pb->var && pa->tiled == pb->tiled &&
+         pa->Tc == pb->Tc && pa->Tr == pb->Tr && pa->clay == pb->clay &&
+         (pa->fa.wt != NULL) == (pb->fa.wt != NULL);
+}
+
 void soi_batch_destroy(soi_batch_t *b)
 {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+  for (int r = 0; r < SOI_BATCH_MAX_BRANCHES; ++r) {
     if (b->branch[r]) (void)hipStreamDestroy(b->branch[r]);
     if (b->join[r]) (void)hipEventDestroy(b->join[r]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->tabs) {
+    for (int g = 0; g < b->ngroups; ++g) (void)hipFree(b->tabs[g]);
+    free(b->tabs);
+  }
   free(b);
 }
 
@@ -1090,21 +1109,90 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
   b->device = device;
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+  /* SOI_BATCH_BRANCHES: concurrent capture branches (default 4) */
+  const char *eb = getenv("SOI_BATCH_BRANCHES");
+  b->nbranch = eb ? atoi(eb) : 4;
+  if (b->nbranch < 1) b->nbranch = 1;
+  if (b->nbranch > SOI_BATCH_MAX_BRANCHES) b->nbranch = SOI_BATCH_MAX_BRANCHES;
+  for (int r = 0; r < b->nbranch; ++r) {
     HIPCHK(hipStreamCreateWithFlags(&b->branch[r], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming));
   }
   /* timing events would become graph nodes: capture with timing off */
   for (int i = 0; i < count; ++i) ((soi_plan_t *)descs[i]->plan)->timing = 0;
+  /* groups of transforms that run as one launch per kernel (blockIdx.y = member): member
+   * pointer tables [x][alpha_tilde][tile buffer] in device memory */
+  int *grp = (int *)malloc(sizeof(int) * count), *gsize = (int *)calloc(count, sizeof(int));
+  int *gfirst = (int *)malloc(sizeof(int) * count);
+  void **htab = (void **)malloc(sizeof(void *) * 3 * count);
+  double *gwork = (double *)calloc(count, sizeof(double)), *load = NULL;
+  if (!grp || !gsize || !gfirst || !htab || !gwork) {
+    free(grp), free(gsize), free(gfirst), free(htab), free(gwork);
+    set_err(SOI_ERR_ALLOC, "host allocation failed");
+    goto fail_quiet;
+  }
+  b->ngroups = 0;
+  for (int i = 0; i < count; ++i) {
+    int g = 0;
+    while (g < b->ngroups && !same_plan(descs[gfirst[g]], descs[i])) ++g;
+    if (g == b->ngroups) gfirst[b->ngroups++] = i;
+    grp[i] = g;
+    gsize[g]++;
+    gwork[g] += (double)descs[i]->N * log2((double)descs[i]->N);
+  }
+  b->tabs = (void **)calloc(b->ngroups, sizeof(void *));
+  load = (double *)calloc(b->nbranch, sizeof(double));
+  if (!b->tabs || !load) {
+    free(grp), free(gsize), free(gfirst), free(htab), free(gwork), free(load);
+    set_err(SOI_ERR_ALLOC, "host allocation failed");
+    goto fail_quiet;
+  }
+  for (int g = 0; g < b->ngroups; ++g) {
+    const int n = gsize[g];
+    for (int i = 0, m = 0; i < count; ++i) {
+      if (grp[i] != g) continue;
+      htab[m] = bufs[i];
+      htab[n + m] = descs[i]->alpha_tilde;
+      htab[2 * n + m] = ((soi_plan_t *)descs[i]->plan)->tbuf;
+      ++m;
+    }
+    hipError_t e = hipMalloc(&b->tabs[g], sizeof(void *) * 3 * n);
+    if (e == hipSuccess)
+      e = hipMemcpy(b->tabs[g], htab, sizeof(void *) * 3 * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      free(grp), free(gsize), free(gfirst), free(htab), free(gwork), free(load);
+      set_err(SOI_ERR_HIP, "batch tables: %s", hipGetErrorString(e));
+      goto fail_quiet;
+    }
+  }
+  free(htab);
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   capturing = 1;
   HIPCHK(hipEventRecord(b->fork, b->stream));
-  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) HIPCHK(hipStreamWaitEvent(b->branch[r], b->fork, 0));
-  for (int i = 0; i < count; ++i) {
-    compute_impl(descs[i], bufs[i], b->branch[i % SOI_BATCH_BRANCHES], 0, 0);
-    if (soi_last_status()) goto fail_quiet;
+  for (int r = 0; r < b->nbranch; ++r) HIPCHK(hipStreamWaitEvent(b->branch[r], b->fork, 0));
+  /* groups in decreasing work, each on the least loaded branch */
+  for (int k = 0; k < b->ngroups; ++k) {
+    int g = -1;
+    for (int h = 0; h < b->ngroups; ++h)
+      if (gwork[h] >= 0 && (g < 0 || gwork[h] > gwork[g])) g = h;
+    int r = 0;
+    for (int q = 1; q < b->nbranch; ++q)
+      if (load[q] < load[r]) r = q;
+    load[r] += gwork[g];
+    gwork[g] = -1;
+    const int n = gsize[g];
+    void *const *tab = (void *const *)b->tabs[g];
+    SoiBatch bt = {tab, tab + n, tab + 2 * n, n};
+    soi_desc_t *d0 = descs[gfirst[g]];
+    soi_plan_t *p0 = (soi_plan_t *)d0->plan;
+    if (run_filter(d0, p0, bufs[gfirst[g]], bt, b->branch[r]) ||
+        run_fft(d0, p0, bufs[gfirst[g]], bt, b->branch[r])) {
+      free(grp), free(gsize), free(gfirst), free(gwork), free(load);
+      goto fail_quiet;
+    }
   }
-  for (int r = 0; r < SOI_BATCH_BRANCHES; ++r) {
+  free(grp), free(gsize), free(gfirst), free(gwork), free(load);
+  for (int r = 0; r < b->nbranch; ++r) {
     HIPCHK(hipEventRecord(b->join[r], b->branch[r]));
     HIPCHK(hipStreamWaitEvent(b->stream, b->join[r], 0));
   }

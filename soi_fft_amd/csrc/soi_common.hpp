@@ -219,6 +219,13 @@ __device__ __forceinline__ void lds_fft(C *buf, int L, int count, int qs, int es
   }
 }
 
+// Buffer of this block's transform in a batched launch (blockIdx.y indexes the table), or the
+// directly passed pointer for a single transform.
+template <typename T> __device__ __forceinline__ T *batch_ptr(T *p, void *const *tab)
+{
+  return tab ? reinterpret_cast<T *>(tab[blockIdx.y]) : p;
+}
+
 // Barrier for LDS hand-off only: waits for this wave's LDS operations, not for its global loads
 // and stores, which stay in flight across it.
 __device__ __forceinline__ void lds_barrier()

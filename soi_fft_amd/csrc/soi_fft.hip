@@ -220,9 +220,11 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
                                                           const C *__restrict__ thi,
                                                           const C *__restrict__ tlo,
                                                           int n2, int lb1, long Mh, int S,
-                                                          C *__restrict__ tout)
+                                                          C *__restrict__ tout, SoiBatch b)
 {
   constexpr int TS = L / E;
+  data = batch_ptr(data, b.at);
+  if constexpr (TILED) tout = batch_ptr(tout, b.tb);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
   const int tid = threadIdx.x;
@@ -272,9 +274,12 @@ __global__ __launch_bounds__(8 * (L / E), fft_minw(BPC, 8 * (L / E))) void k_fft
                                                           C *__restrict__ X,
                                                           const C *__restrict__ tw,
                                                           const double2 *__restrict__ winv,
-                                                          int n1, long Mh, long M, int S)
+                                                          int n1, long Mh, long M, int S,
+                                                          SoiBatch b)
 {
   constexpr int T = 8, TS = L / E;
+  data = batch_ptr(data, b.tb);
+  X = batch_ptr(X, b.x);
   using LY = QFast<L, T>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
@@ -314,9 +319,12 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
                                                           C *__restrict__ X,
                                                           const C *__restrict__ tw,
                                                           const double2 *__restrict__ winv,
-                                                          int n1, long Mh, long M, int S)
+                                                          int n1, long Mh, long M, int S,
+                                                          SoiBatch b)
 {
   constexpr int TS = L / E;
+  data = batch_ptr(data, b.at);
+  X = batch_ptr(X, b.x);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real_t<C> *ldsd = reinterpret_cast<real_t<C> *>(lds_raw);
   const int tid = threadIdx.x;
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 // ---------------------------------------------------------------------------------------------
 template <int L, int E, int T, int NQP, int BPC, typename RS, typename LY, typename C>
 static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
-                       int n2, int lb1, long Mh, int S, C *tout, hipStream_t st)
+                       int n2, int lb1, long Mh, int S, C *tout, SoiBatch b, hipStream_t st)
 {
   if (tout) {
     auto kt = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, true>;
@@ -393,8 +401,8 @@ static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
       (void)hipFuncSetAttribute((const void *)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       once_t = true;
     }
-    hipLaunchKernelGGL(kt, dim3((unsigned)((n2 / T) * S)), dim3(T * (L / E)), lds, st, data, tw,
-                       thi, tlo, n2, lb1, Mh, S, tout);
+    hipLaunchKernelGGL(kt, dim3((unsigned)((n2 / T) * S), b.n > 0 ? b.n : 1), dim3(T * (L / E)),
+                       lds, st, data, tw, thi, tlo, n2, lb1, Mh, S, tout, b);
     return (int)hipGetLastError();
   }
   auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, false>;
@@ -405,14 +413,14 @@ static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     once = true;
   }
-  hipLaunchKernelGGL(kern, dim3((unsigned)((n2 / T) * S)), dim3(T * (L / E)), lds, st, data, tw,
-                     thi, tlo, n2, lb1, Mh, S, (C *)nullptr);
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n2 / T) * S), b.n > 0 ? b.n : 1), dim3(T * (L / E)),
+                     lds, st, data, tw, thi, tlo, n2, lb1, Mh, S, (C *)nullptr, b);
   return (int)hipGetLastError();
 }
 
 template <int L, int E, int T, int NQP, int BPC, typename RS, typename C>
 static int launch_rows(const C *data, C *X, const C *tw, const double2 *winv,
-                       int n1, long Mh, long M, int S, hipStream_t st)
+                       int n1, long Mh, long M, int S, SoiBatch b, hipStream_t st)
 {
   auto kern = k_fft_rows<L, E, T, NQP, BPC, RS, C>;
   constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * sizeof(real_t<C>);
@@ -422,14 +430,14 @@ static int launch_rows(const C *data, C *X, const C *tw, const double2 *winv,
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     once = true;
   }
-  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / T) * S)), dim3(T * (L / E)), lds, st, data, X,
-                     tw, winv, n1, Mh, M, S);
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / T) * S), b.n > 0 ? b.n : 1), dim3(T * (L / E)),
+                     lds, st, data, X, tw, winv, n1, Mh, M, S, b);
   return (int)hipGetLastError();
 }
 
 template <int L, int E, int BPC, typename RS, typename C>
 static int launch_rows_t(const C *data, C *X, const C *tw, const double2 *winv, int n1, long Mh,
-                         long M, int S, hipStream_t st)
+                         long M, int S, SoiBatch b, hipStream_t st)
 {
   auto kern = k_fft_rows_t<L, E, BPC, RS, C>;
   constexpr size_t lds = (size_t)QFast<L, 8>::size * sizeof(real_t<C>);
@@ -439,8 +447,8 @@ static int launch_rows_t(const C *data, C *X, const C *tw, const double2 *winv, 
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     once = true;
   }
-  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / 8) * S)), dim3(8 * (L / E)), lds, st, data, X,
-                     tw, winv, n1, Mh, M, S);
+  hipLaunchKernelGGL(kern, dim3((unsigned)((n1 / 8) * S), b.n > 0 ? b.n : 1), dim3(8 * (L / E)),
+                     lds, st, data, X, tw, winv, n1, Mh, M, S, b);
   return (int)hipGetLastError();
 }
 
@@ -469,12 +477,12 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
 // T = 8, complex64 only).
 template <typename C>
 static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int lb1,
-                    long Mh, int S, int T, int variant, C *tout, hipStream_t st)
+                    long Mh, int S, int T, int variant, C *tout, SoiBatch b, hipStream_t st)
 {
   if (n2 % T) return (int)hipErrorInvalidValue;
 #define COLS(VV, LV, EV, TV, NQ, BPC, LY, ...) \
   if (n1 == LV && T == TV && variant == VV) \
-    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>, C>(data, tw1, thi, tlo, n2, lb1, Mh, S, tout, st);
+    return launch_cols<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, LY<LV, TV / NQ>, C>(data, tw1, thi, tlo, n2, lb1, Mh, S, tout, b, st);
   COLS(0, 256, 16, 8, 1, 1, QFast, 16, 16) COLS(0, 320, 20, 8, 1, 1, QFast, 10, 4, 4, 2)
   COLS(0, 512, 16, 8, 1, 1, QFast, 16, 16, 2) COLS(0, 640, 20, 8, 1, 1, QFast, 10, 4, 4, 4)
   COLS(0, 1024, 16, 8, 1, 1, QFast, 16, 16, 4) COLS(0, 1280, 20, 8, 1, 1, QFast, 10, 4, 4, 4, 2)
@@ -492,11 +500,11 @@ static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, i
 
 template <typename C>
 static int rows_reg(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2,
-                    long Mh, long M, int S, int T, hipStream_t st)
+                    long Mh, long M, int S, int T, SoiBatch b, hipStream_t st)
 {
   if (n1 % T) return (int)hipErrorInvalidValue;
 #define ROWS(LV, EV, TV, NQ, BPC, ...) \
-  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, st);
+  if (n2 == LV && T == TV) return launch_rows<LV, EV, TV, NQ, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, b, st);
   ROWS(256, 16, 8, 1, 1, 16, 16) ROWS(320, 20, 8, 1, 1, 10, 4, 4, 2) ROWS(512, 16, 8, 1, 1, 16, 16, 2)
   ROWS(640, 20, 8, 1, 1, 10, 4, 4, 4) ROWS(1024, 16, 8, 1, 1, 16, 16, 4)
   ROWS(1280, 20, 8, 1, 1, 10, 4, 4, 4, 2) ROWS(2048, 16, 8, 1, 1, 16, 16, 8)
@@ -509,11 +517,11 @@ static int rows_reg(const C *data, C *X, const C *tw2, const double2 *winv, int 
 
 template <typename C>
 static int rows_t(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2, long Mh,
-                  long M, int S, hipStream_t st)
+                  long M, int S, SoiBatch b, hipStream_t st)
 {
   if (n1 % 8) return (int)hipErrorInvalidValue;
 #define ROWST(LV, EV, BPC, ...) \
-  if (n2 == LV) return launch_rows_t<LV, EV, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, st);
+  if (n2 == LV) return launch_rows_t<LV, EV, BPC, Radices<__VA_ARGS__>, C>(data, X, tw2, winv, n1, Mh, M, S, b, st);
   ROWST(256, 16, 1, 16, 16) ROWST(320, 20, 1, 10, 4, 4, 2) ROWST(512, 16, 1, 16, 16, 2)
   ROWST(640, 20, 1, 10, 4, 4, 4) ROWST(1024, 16, 1, 16, 16, 4)
   ROWST(1280, 20, 1, 10, 4, 4, 4, 2) ROWST(2048, 16, 1, 16, 16, 8)
@@ -533,43 +541,43 @@ extern "C" int soik_fft_rows_tiled_supported(int n2, int f32)
 
 extern "C" int soik_fft_cols_reg(double2 *data, const double2 *tw1, const double2 *thi,
                                  const double2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                 int T, int variant, double2 *tout, hipStream_t st)
+                                 int T, int variant, double2 *tout, SoiBatch b, hipStream_t st)
 {
-  return cols_reg<double2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, st);
+  return cols_reg<double2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, b, st);
 }
 
 extern "C" int soik_fft_rows_tiled(const double2 *data, double2 *X, const double2 *tw2,
                                    const double2 *winv, int n1, int n2, long Mh, long M, int S,
-                                   hipStream_t st)
+                                   SoiBatch b, hipStream_t st)
 {
-  return rows_t<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, st);
+  return rows_t<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, b, st);
 }
 
 extern "C" int soik_fft_rows_tiled_f32(const float2 *data, float2 *X, const float2 *tw2,
                                        const double2 *winv, int n1, int n2, long Mh, long M,
-                                       int S, hipStream_t st)
+                                       int S, SoiBatch b, hipStream_t st)
 {
-  return rows_t<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, st);
+  return rows_t<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, b, st);
 }
 
 extern "C" int soik_fft_rows_reg(const double2 *data, double2 *X, const double2 *tw2,
                                  const double2 *winv, int n1, int n2, long Mh, long M, int S,
-                                 int T, hipStream_t st)
+                                 int T, SoiBatch b, hipStream_t st)
 {
-  return rows_reg<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, st);
+  return rows_reg<double2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, b, st);
 }
 
 // FP32 form: complex64 data and tables (rounded from their FP64 values); W_inv stays FP64.
 extern "C" int soik_fft_cols_reg_f32(float2 *data, const float2 *tw1, const float2 *thi,
                                      const float2 *tlo, int n1, int n2, int lb1, long Mh, int S,
-                                     int T, int variant, float2 *tout, hipStream_t st)
+                                     int T, int variant, float2 *tout, SoiBatch b, hipStream_t st)
 {
-  return cols_reg<float2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, st);
+  return cols_reg<float2>(data, tw1, thi, tlo, n1, n2, lb1, Mh, S, T, variant, tout, b, st);
 }
 
 extern "C" int soik_fft_rows_reg_f32(const float2 *data, float2 *X, const float2 *tw2,
                                      const double2 *winv, int n1, int n2, long Mh, long M, int S,
-                                     int T, hipStream_t st)
+                                     int T, SoiBatch b, hipStream_t st)
 {
-  return rows_reg<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, st);
+  return rows_reg<float2>(data, X, tw2, winv, n1, n2, Mh, M, S, T, b, st);
 }

@@ -289,9 +289,11 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
                              (FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C>::MINW)) void k_filter(
     const C *__restrict__ x, C *__restrict__ at, const real_t<C> *__restrict__ taps,
     const C *__restrict__ post, const C *__restrict__ twS, long M, long Mh, long nJ,
-    int tiles_per_block)
+    int tiles_per_block, SoiBatch b)
 {
   typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C> CF;
+  x = batch_ptr(x, b.x);
+  at = batch_ptr(at, b.at);
   typedef real_t<C> RT;
   __shared__ C bufA[CF::BUFN];
   __shared__ C bufB[CF::DB ? CF::BUFN : 1];
@@ -416,9 +418,11 @@ template <int S>
 __global__ __launch_bounds__(MfmaCfg<S>::T, MfmaCfg<S>::T / 256) void k_filter_mfma(
     const float2 *__restrict__ x, float2 *__restrict__ at, const bf16x8 *__restrict__ wt,
     const float2 *__restrict__ post, const float2 *__restrict__ twS, long M, long Mh, long nJ,
-    int tiles_per_block)
+    int tiles_per_block, SoiBatch b)
 {
   typedef MfmaCfg<S> CF;
+  x = batch_ptr(x, b.x);
+  at = batch_ptr(at, b.at);
   constexpr int LPW = CF::LPW, NCH = CF::NCH, RP = CF::RP;
   __shared__ __align__(16) float2 raw[CF::XN];
   __shared__ __align__(16) __bf16 As[S * 6 * RP];
@@ -585,9 +589,10 @@ template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(k_filter_mfma<S>, dim3((unsigned)nb), dim3(CF::T), 0, st,
-                     (const float2 *)a->x, (float2 *)a->at, (const bf16x8 *)a->wt,
-                     (const float2 *)a->post, (const float2 *)a->twS, a->M, a->Mh, a->nJ, tpb);
+  hipLaunchKernelGGL(k_filter_mfma<S>, dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1), dim3(CF::T), 0,
+                     st, (const float2 *)a->x, (float2 *)a->at, (const bf16x8 *)a->wt,
+                     (const float2 *)a->post, (const float2 *)a->twS, a->M, a->Mh, a->nJ, tpb,
+                     a->b);
   return (int)hipGetLastError();
 }
 
@@ -639,9 +644,9 @@ static int launch1(const SoiFilterArgs *a, hipStream_t st)
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * CF::NG)), dim3(CF::T), 0, st,
-                     (const C *)a->x, (C *)a->at, (const real_t<C> *)a->taps,
-                     (const C *)a->post, (const C *)a->twS, a->M, a->Mh, a->nJ, tpb);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * CF::NG), a->b.n > 0 ? a->b.n : 1), dim3(CF::T), 0,
+                     st, (const C *)a->x, (C *)a->at, (const real_t<C> *)a->taps,
+                     (const C *)a->post, (const C *)a->twS, a->M, a->Mh, a->nJ, tpb, a->b);
   return (int)hipGetLastError();
 }
 

@@ -19,6 +19,16 @@ typedef struct { double x, y; } soi_c64_t;
 typedef struct { float x, y; } soi_c32_t;
 #endif
 
+/* Batched launch: n transforms of one shape in one launch, blockIdx.y = transform.  The tables
+ * are device arrays of the per-transform buffers (in/out vector, alpha_tilde, pass-1 tile
+ * buffer).  n = 0: one transform on the pointers passed directly. */
+typedef struct SoiBatch {
+  void *const *x;
+  void *const *at;
+  void *const *tb;
+  int n;
+} SoiBatch;
+
 typedef struct SoiFilterArgs {
   const void *x;      /* input [M][S], complex128 (f32 = 0) or complex64 (f32 = 1) */
   void *at;           /* output alpha_tilde [S][M_hat], same element type */
@@ -28,6 +38,7 @@ typedef struct SoiFilterArgs {
   const void *wt;     /* complex64 only: bf16 tap pieces of the MFMA form (NULL: VALU form) */
   long M, Mh, nJ;
   int S, n_mu, d_mu, BP, NP, NT, U, W, ct, f32;
+  SoiBatch b;         /* batched launch tables (x -> b.x[t], at -> b.at[t]) */
 } SoiFilterArgs;
 
 int soik_make_tables(void *taps, soi_c64_t *post, soi_c64_t *w_full, soi_c64_t *winv, int S,
@@ -42,42 +53,42 @@ int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD);
 size_t soik_filter_mfma_table_bytes(int S, int BPAD);
 int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, hipStream_t st);
 int soik_fft_single(const soi_c64_t *at, soi_c64_t *X, const soi_c64_t *tw,
-                    const soi_c64_t *winv, long L, long M, int S, int count, hipStream_t st);
+                    const soi_c64_t *winv, long L, long M, int S, int count, SoiBatch b, hipStream_t st);
 int soik_fft_col(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                  const soi_c64_t *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
-                 hipStream_t st);
+                 SoiBatch b, hipStream_t st);
 int soik_fft_row(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                  const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
-                 hipStream_t st);
+                 SoiBatch b, hipStream_t st);
 int soik_fft_single_f32(const soi_c32_t *at, soi_c32_t *X, const soi_c32_t *tw,
-                        const soi_c64_t *winv, long L, long M, int S, int count, hipStream_t st);
+                        const soi_c64_t *winv, long L, long M, int S, int count, SoiBatch b, hipStream_t st);
 int soik_fft_col_f32(soi_c32_t *data, const soi_c32_t *tw1, const soi_c32_t *thi,
                      const soi_c32_t *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
-                     hipStream_t st);
+                     SoiBatch b, hipStream_t st);
 int soik_fft_row_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
                      const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
-                     hipStream_t st);
+                     SoiBatch b, hipStream_t st);
 int soik_fft_reg_supported(int L, int rows);
 int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                       const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
-                      int layout, soi_c64_t *tout, hipStream_t st);
+                      int layout, soi_c64_t *tout, SoiBatch b, hipStream_t st);
 int soik_fft_rows_reg(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                       const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
-                      hipStream_t st);
+                      SoiBatch b, hipStream_t st);
 int soik_fft_cols_reg_f32(soi_c32_t *data, const soi_c32_t *tw1, const soi_c32_t *thi,
                           const soi_c32_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
-                          int layout, soi_c32_t *tout, hipStream_t st);
+                          int layout, soi_c32_t *tout, SoiBatch b, hipStream_t st);
 /* pass 2 over pass 1's 8-row tiles (tout of the column pass) */
 int soik_fft_rows_tiled_supported(int n2, int f32);
 int soik_fft_rows_tiled(const soi_c64_t *data, soi_c64_t *X, const soi_c64_t *tw2,
                         const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S,
-                        hipStream_t st);
+                        SoiBatch b, hipStream_t st);
 int soik_fft_rows_tiled_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
                             const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S,
-                            hipStream_t st);
+                            SoiBatch b, hipStream_t st);
 int soik_fft_rows_reg_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
                           const soi_c64_t *winv, int n1, int n2, long Mh, long M, int S, int T,
-                          hipStream_t st);
+                          SoiBatch b, hipStream_t st);
 int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st);
 int soik_compress(int *out, long stride, const double *x, int len, int nchunk, int e_max,
                   const int *emi, int *nwords, hipStream_t st);

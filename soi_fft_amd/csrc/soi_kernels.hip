@@ -141,8 +141,10 @@ template <typename C>
 __global__ __launch_bounds__(256) void k_fft_single(const C *__restrict__ at, C *__restrict__ X,
                                                     const C *__restrict__ tw,
                                                     const double2 *__restrict__ winv, long L,
-                                                    long M, int S, int count)
+                                                    long M, int S, int count, SoiBatch b)
 {
+  at = batch_ptr(at, b.at);
+  X = batch_ptr(X, b.x);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -168,8 +170,9 @@ template <typename C>
 __global__ __launch_bounds__(256) void k_fft_col(C *__restrict__ data, const C *__restrict__ tw1,
                                                  const C *__restrict__ thi,
                                                  const C *__restrict__ tlo, int n1, int n2,
-                                                 int Tc, int B1, long Mh, int S)
+                                                 int Tc, int B1, long Mh, int S, SoiBatch b)
 {
+  data = batch_ptr(data, b.at);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -201,8 +204,11 @@ template <typename C>
 __global__ __launch_bounds__(256) void k_fft_row(const C *__restrict__ data, C *__restrict__ X,
                                                  const C *__restrict__ tw2,
                                                  const double2 *__restrict__ winv, int n1,
-                                                 int n2, int Tr, long Mh, long M, int S)
+                                                 int n2, int Tr, long Mh, long M, int S,
+                                                 SoiBatch b)
 {
+  data = batch_ptr(data, b.at);
+  X = batch_ptr(X, b.x);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -504,83 +510,83 @@ static void set_lds_attr(const void *f)
 
 template <typename C>
 static int fft_single(const C *at, C *X, const C *tw, const double2 *winv, long L, long M, int S,
-                      int count, hipStream_t st)
+                      int count, SoiBatch b, hipStream_t st)
 {
   static bool once = false;
   if (!once) { set_lds_attr((const void *)k_fft_single<C>); once = true; }
   size_t lds = (size_t)count * (L + 1) * sizeof(C);
-  hipLaunchKernelGGL(k_fft_single<C>, dim3((S + count - 1) / count), dim3(256), lds, st, at, X,
-                     tw, winv, L, M, S, count);
+  hipLaunchKernelGGL(k_fft_single<C>, dim3((S + count - 1) / count, b.n > 0 ? b.n : 1), dim3(256),
+                     lds, st, at, X, tw, winv, L, M, S, count, b);
   return (int)hipGetLastError();
 }
 
 template <typename C>
 static int fft_col(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int Tc,
-                   int B1, long Mh, int S, hipStream_t st)
+                   int B1, long Mh, int S, SoiBatch b, hipStream_t st)
 {
   static bool once = false;
   if (!once) { set_lds_attr((const void *)k_fft_col<C>); once = true; }
   size_t lds = (size_t)n1 * Tc * sizeof(C);
   unsigned grid = (unsigned)((n2 / Tc) * S);
-  hipLaunchKernelGGL(k_fft_col<C>, dim3(grid), dim3(256), lds, st, data, tw1, thi, tlo, n1, n2,
-                     Tc, B1, Mh, S);
+  hipLaunchKernelGGL(k_fft_col<C>, dim3(grid, b.n > 0 ? b.n : 1), dim3(256), lds, st, data, tw1,
+                     thi, tlo, n1, n2, Tc, B1, Mh, S, b);
   return (int)hipGetLastError();
 }
 
 template <typename C>
 static int fft_row(const C *data, C *X, const C *tw2, const double2 *winv, int n1, int n2,
-                   int Tr, long Mh, long M, int S, hipStream_t st)
+                   int Tr, long Mh, long M, int S, SoiBatch b, hipStream_t st)
 {
   static bool once = false;
   if (!once) { set_lds_attr((const void *)k_fft_row<C>); once = true; }
   size_t lds = (size_t)n2 * Tr * sizeof(C);
   unsigned grid = (unsigned)((n1 / Tr) * S);
-  hipLaunchKernelGGL(k_fft_row<C>, dim3(grid), dim3(256), lds, st, data, X, tw2, winv, n1, n2,
-                     Tr, Mh, M, S);
+  hipLaunchKernelGGL(k_fft_row<C>, dim3(grid, b.n > 0 ? b.n : 1), dim3(256), lds, st, data, X,
+                     tw2, winv, n1, n2, Tr, Mh, M, S, b);
   return (int)hipGetLastError();
 }
 
 extern "C" int soik_fft_single(const double2 *at, double2 *X, const double2 *tw,
                                const double2 *winv, long L, long M, int S, int count,
-                               hipStream_t st)
+                               SoiBatch b, hipStream_t st)
 {
-  return fft_single<double2>(at, X, tw, winv, L, M, S, count, st);
+  return fft_single<double2>(at, X, tw, winv, L, M, S, count, b, st);
 }
 
 extern "C" int soik_fft_col(double2 *data, const double2 *tw1, const double2 *thi,
                             const double2 *tlo, int n1, int n2, int Tc, int B1, long Mh, int S,
-                            hipStream_t st)
+                            SoiBatch b, hipStream_t st)
 {
-  return fft_col<double2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, st);
+  return fft_col<double2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, b, st);
 }
 
 extern "C" int soik_fft_row(const double2 *data, double2 *X, const double2 *tw2,
                             const double2 *winv, int n1, int n2, int Tr, long Mh, long M, int S,
-                            hipStream_t st)
+                            SoiBatch b, hipStream_t st)
 {
-  return fft_row<double2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, st);
+  return fft_row<double2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, b, st);
 }
 
 // FP32 form: complex64 data and tables; W_inv stays FP64
 extern "C" int soik_fft_single_f32(const float2 *at, float2 *X, const float2 *tw,
                                    const double2 *winv, long L, long M, int S, int count,
-                                   hipStream_t st)
+                                   SoiBatch b, hipStream_t st)
 {
-  return fft_single<float2>(at, X, tw, winv, L, M, S, count, st);
+  return fft_single<float2>(at, X, tw, winv, L, M, S, count, b, st);
 }
 
 extern "C" int soik_fft_col_f32(float2 *data, const float2 *tw1, const float2 *thi,
                                 const float2 *tlo, int n1, int n2, int Tc, int B1, long Mh,
-                                int S, hipStream_t st)
+                                int S, SoiBatch b, hipStream_t st)
 {
-  return fft_col<float2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, st);
+  return fft_col<float2>(data, tw1, thi, tlo, n1, n2, Tc, B1, Mh, S, b, st);
 }
 
 extern "C" int soik_fft_row_f32(const float2 *data, float2 *X, const float2 *tw2,
                                 const double2 *winv, int n1, int n2, int Tr, long Mh, long M,
-                                int S, hipStream_t st)
+                                int S, SoiBatch b, hipStream_t st)
 {
-  return fft_row<float2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, st);
+  return fft_row<float2>(data, X, tw2, winv, n1, n2, Tr, Mh, M, S, b, st);
 }
 
 extern "C" int soik_max_exponent(const double *x, int len, int nchunk, int *out, hipStream_t st)

@@ -169,10 +169,12 @@ int soi_autotune(soi_desc_t *d, cfft_complex_t *scratch, int reps);
 
 /* ---- batching form ----------------------------------------------------------------------
  * Many independent transforms (initialised descriptors of any shapes and either precision,
- * without use_vlc), each with its own device buffer, captured once into one HIP graph whose
- * branches run concurrently; soi_batch_run replays it with one launch (in place on the same
- * buffers, which the caller refills between runs).  With autotune != 0 each distinct shape is
- * tuned once (soi_autotune) before capture.  Returns NULL on error (soi_last_error). */
+ * without use_vlc), each with its own device buffer, captured once into one HIP graph:
+ * transforms of the same shape run as one launch per kernel, and those groups run on concurrent
+ * graph branches; soi_batch_run replays it with one launch (in place on the same buffers, which
+ * the caller refills between runs).  With autotune != 0 each distinct shape is tuned once
+ * (soi_autotune) before capture.  SOI_BATCH_BRANCHES (1..16, default 4) sets the branch count.
+ * Returns NULL on error (soi_last_error). */
 typedef struct soi_batch_s soi_batch_t;
 soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int count, int autotune);
 int soi_batch_run(soi_batch_t *b, void *stream);   /* NULL = the batch's own stream */

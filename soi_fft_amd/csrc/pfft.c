@@ -201,6 +201,12 @@ static void fft_variants(soi_plan_t *pl)
   const int n1 = pl->n1, n2 = pl->n2;
   const int c4 = n1 == 1024 || n1 == 1280 || n1 == 2048 || n1 == 2560;
   cols[nc][0] = pl->Tc, cols[nc][1] = pl->clay, ++nc;
+  if (soik_fft_cols_short_tile(n1)) { /* short columns: one configuration */
+    pl->nvar = 1;
+    pl->vtab[0][0] = pl->Tc, pl->vtab[0][1] = pl->clay, pl->vtab[0][2] = pl->Tr;
+    pl->var = 0;
+    return;
+  }
   if (n2 % 8 == 0 && !(pl->Tc == 8 && pl->clay == 0)) cols[nc][0] = 8, cols[nc][1] = 0, ++nc;
   if (c4 && n2 % 4 == 0 && !(pl->Tc == 4 && pl->clay == 0)) cols[nc][0] = 4, cols[nc][1] = 0, ++nc;
   if ((n1 == 2048 || n1 == 2560) && n2 % 8 == 0 && pl->clay != 1)
@@ -246,11 +252,13 @@ static int plan_fft(soi_plan_t *pl)
     /* SOI_FFT_N1 forces the column length (tuning) */
     const char *en = getenv("SOI_FFT_N1");
     long force = en ? atol(en) : 0;
-    for (long n1 = 8; n1 <= 4096; n1 += 8) {
+    for (long n1 = 2; n1 <= 4096; ++n1) {
       if (Mh % n1) continue;
       long n2 = Mh / n1;
-      if (n2 % 8 || n2 > 4096) continue;
+      const int st = soik_fft_cols_short_tile((int)n1);
+      if (n2 % (st ? st : 8) || n2 > 4096) continue;
       if (!soik_fft_reg_supported((int)n1, 0) || !soik_fft_reg_supported((int)n2, 1)) continue;
+      if (st && n1 % 8 && n2 != 256) continue; /* 10-wide rows exist for 256 points only */
       long cost = (n1 > n2 ? n1 : n2);
       if (force == n1) cost = 0;
       if (cost <= best) {   /* ties: the longer column */
@@ -269,12 +277,17 @@ static int plan_fft(soi_plan_t *pl)
       if (er && (atoi(er) == 4 || atoi(er) == 8)) pl->Tr = atoi(er);
       if (pl->n2 % pl->Tc || pl->n1 % pl->Tr) pl->Tc = pl->Tr = 8;
       if (pl->n2 == 2560) pl->Tr = 4; /* 2560-point rows exist with 4-wide tiles only */
+      const int stile = soik_fft_cols_short_tile(pl->n1);
+      if (stile) { /* short columns: their own tile width; rows 10-wide unless n1 % 8 == 0 */
+        pl->Tc = stile;
+        pl->Tr = pl->n1 % 8 ? 10 : 8;
+      }
       /* column-pass kernel variant: 1 = one padded LDS run per sequence (8-wide tiles of 2048
        * or 2560); 2 = one 4-wide block per CU with the whole register budget (2560, the
        * default there: 20 values per thread do not fit 1024-thread blocks); SOI_FFT_CVAR
        * overrides for tuning */
       const char *el = getenv("SOI_FFT_CVAR");
-      int cv = el ? atoi(el) : (pl->n1 == 2560 ? 2 : 0);
+      int cv = el && !stile ? atoi(el) : (pl->n1 == 2560 ? 2 : 0);
       pl->clay = 0;
       if (cv == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) pl->clay = 1;
       if (cv == 2 && pl->n1 == 2560) {

@@ -464,8 +464,22 @@ extern "C" int soik_fft_reg_supported(int L, int rows)
       return 1;
     case 2560:
       return 1;
+    case 20: case 30: case 40: case 50: case 60: case 80: case 100: case 160: case 200:
+      return !rows;  // short columns only
     default:
       return 0;
+  }
+}
+
+// Column tile width of the short column lengths (0: a regular length, tile 8 or 4)
+extern "C" int soik_fft_cols_short_tile(int L)
+{
+  switch (L) {
+    case 20: case 30: return 64;
+    case 40: case 60: return 32;
+    case 50: case 80: case 100: return 16;
+    case 160: case 200: return 8;
+    default: return 0;
   }
 }
 
@@ -494,6 +508,13 @@ static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, i
   if constexpr (sizeof(C) == 8) {  // complex64: the radix-10 stage last needs fewer registers
     COLS(3, 2560, 20, 8, 2, 1, QFast, 4, 4, 4, 4, 10)
   }
+  // short columns (M_hat = n1 x 256 with n1 of 2^a 3^b 5^c form): wide tiles, one block of
+  // 64 - 80 threads over 8 - 64 columns
+  COLS(0, 20, 20, 64, 1, 1, QFast, 10, 2) COLS(0, 30, 30, 64, 1, 1, QFast, 10, 3)
+  COLS(0, 40, 20, 32, 1, 1, QFast, 10, 4) COLS(0, 50, 10, 16, 1, 1, QFast, 10, 5)
+  COLS(0, 60, 30, 32, 1, 1, QFast, 10, 3, 2) COLS(0, 80, 20, 16, 1, 1, QFast, 10, 4, 2)
+  COLS(0, 100, 20, 16, 1, 1, QFast, 10, 10) COLS(0, 160, 20, 8, 1, 1, QFast, 10, 4, 4)
+  COLS(0, 200, 20, 8, 1, 1, QFast, 10, 10, 2)
 #undef COLS
   return (int)hipErrorInvalidValue;
 }
@@ -511,6 +532,7 @@ static int rows_reg(const C *data, C *X, const C *tw2, const double2 *winv, int 
   ROWS(2560, 20, 4, 1, 1, 10, 4, 4, 4, 4)
   ROWS(1024, 16, 4, 1, 2, 16, 16, 4) ROWS(1280, 20, 4, 1, 2, 10, 4, 4, 4, 2)
   ROWS(2048, 16, 4, 1, 2, 16, 16, 8)
+  ROWS(256, 16, 10, 1, 1, 16, 16)  // rows over short columns whose count is not a multiple of 8
 #undef ROWS
   return (int)hipErrorInvalidValue;
 }

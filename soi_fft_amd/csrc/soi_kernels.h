@@ -69,6 +69,7 @@ int soik_fft_row_f32(const soi_c32_t *data, soi_c32_t *X, const soi_c32_t *tw2,
                      const soi_c64_t *winv, int n1, int n2, int Tr, long Mh, long M, int S,
                      SoiBatch b, hipStream_t st);
 int soik_fft_reg_supported(int L, int rows);
+int soik_fft_cols_short_tile(int L);
 int soik_fft_cols_reg(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,
                       const soi_c64_t *tlo, int n1, int n2, int lb1, long Mh, int S, int T,
                       int layout, soi_c64_t *tout, SoiBatch b, hipStream_t st);

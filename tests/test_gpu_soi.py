@@ -319,3 +319,40 @@ def test_fft_kernel_variants(soi, torch_cuda, monkeypatch, N, P, env):
     err = (torch.linalg.norm(outs[1] - outs[0]) / torch.linalg.norm(outs[0])).item()
     assert err < (1e-15 if "SOI_FFT_N1" not in env else 3e-15)
     assert (torch.linalg.norm(outs[1] - ref) / torch.linalg.norm(ref)).item() < 3e-14
+
+
+SHORT_COLUMNS = [20, 30, 40, 50, 60, 80, 100, 160, 200]
+
+
+@pytest.mark.parametrize("n1", SHORT_COLUMNS)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_short_column_splits(soi, orc, torch_cuda, n1, prec):
+    """M_hat = n1 x 256 with the short column lengths of the register FFT (wide column tiles;
+    10-wide row tiles where n1 is not a multiple of 8): P = 8, N = 8 * (4/5) * M_hat, in both
+    precisions against the oracle."""
+    P = 8
+    M = 4 * n1 * 256 // 5
+    N = M * P
+    print(f"case n1={n1} N={N} P={P} prec={prec}", flush=True)
+    x = orc.uniform_input(N, 404)
+    p = orc.params(N, P)
+    if prec:
+        x32 = x.astype(np.complex64)
+        ref, _ = orc.compute_soi(p, x32.astype(np.complex128))
+        d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+        try:
+            xt = to_dev(torch_cuda, x32)
+            torch_cuda.cuda.synchronize()
+            soi.compute_soi(d, xt.data_ptr())
+            got = xt.cpu().numpy().astype(np.complex128)
+        finally:
+            soi.free_soi_descriptor(d)
+        err = rel(got, ref)
+        print(f"  fp32 rel-L2 {err:.3e}", flush=True)
+        assert err < 1e-6
+    else:
+        ref, _ = orc.compute_soi(p, x)
+        got, _ = run_soi(soi, torch_cuda, x, P)
+        err = rel(got, ref)
+        print(f"  fp64 rel-L2 {err:.3e}", flush=True)
+        assert err < parity_tol(orc, p)

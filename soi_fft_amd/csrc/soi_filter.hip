@@ -212,12 +212,14 @@ template <typename CF, int S, int NMU, int NT, int DMU, int BP, int U, int NP, b
           typename C>
 __device__ __forceinline__ void filter_tile(const FilterThread<BP, CT, C> &th, C *cur, C *nxt,
                                             const C *tw, const C *__restrict__ x,
-                                            C *__restrict__ at, long tile, bool more, long N,
-                                            long Mh, long nJ)
+                                            const C *__restrict__ xn, C *__restrict__ at,
+                                            long tile, long tnext, bool more, long N, long Mh,
+                                            long nJ)
 {
+  // x: this tile's input; xn, tnext: the next tile's input and index (prefetched when more)
   const long J0 = tile * CF::TJ;
   if constexpr (CF::DB) {
-    if (more) dma_rows<CF, C>(nxt, x, (tile + 1) * CF::TJ * DMU * S, N, th.wave, th.lane);
+    if (more) dma_rows<CF, C>(nxt, xn, tnext * CF::TJ * DMU * S, N, th.wave, th.lane);
   } else {
     dma_rows<CF, C>(cur, x, tile * CF::TJ * DMU * S, N, th.wave, th.lane);
     wait_vmcnt<0>();
@@ -277,14 +279,31 @@ __device__ __forceinline__ void filter_tile(const FilterThread<BP, CT, C> &th, C
   }
   if constexpr (CF::DB) {
     // this wave's prefetch loads are complete once at most its stores are outstanding (the
-    // stores were issued after them; a tile with a successor stores every row)
+    // stores were issued after them; a whole tile stores every row, the last partial tile of a
+    // transform fewer)
     asm volatile("" ::: "memory");
-    if (more) wait_vmcnt<CF::NSC_MIN>();
+    if (more) {
+      if (J0 + CF::TJ <= nJ) wait_vmcnt<CF::NSC_MIN>();
+      else wait_vmcnt<0>();
+    }
   }
   lds_barrier();
 }
 
-template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C>
+// Input and output of member m of a batched launch (the pointers passed directly for one
+// transform)
+template <typename C>
+__device__ __forceinline__ const C *member_in(const C *x, const SoiBatch &b, long m)
+{
+  return b.x ? reinterpret_cast<const C *>(b.x[m]) : x;
+}
+template <typename C> __device__ __forceinline__ C *member_out(C *at, const SoiBatch &b, long m)
+{
+  return b.at ? reinterpret_cast<C *>(b.at[m]) : at;
+}
+
+template <int S, int NMU, int NT, int DMU, int BP, int U, int NP, int W, bool CT, typename C,
+          bool BATCHED>
 __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
                              (FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C>::MINW)) void k_filter(
     const C *__restrict__ x, C *__restrict__ at, const real_t<C> *__restrict__ taps,
@@ -292,8 +311,6 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
     int tiles_per_block, SoiBatch b)
 {
   typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C> CF;
-  x = batch_ptr(x, b.x);
-  at = batch_ptr(at, b.at);
   typedef real_t<C> RT;
   __shared__ C bufA[CF::BUFN];
   __shared__ C bufB[CF::DB ? CF::BUFN : 1];
@@ -342,24 +359,60 @@ __global__ __launch_bounds__(filter_threads(S, NT, NP, W),
   for (int e = th.tid; e < S; e += CF::T) tw[e] = twS[e];
 
   const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
-  const long t0 = (long)bg * tiles_per_block;
-  const long t1 = min(ntiles, t0 + tiles_per_block);
   const long N = M * S;
-
-  if constexpr (CF::DB) {
-    if (t0 < t1) dma_rows<CF, C>(bufA, x, t0 * CF::TJ * DMU * S, N, th.wave, th.lane);
-    wait_vmcnt<0>();
-    lds_barrier();
-    for (long tile = t0; tile < t1; ++tile) {
-      const bool odd = (tile - t0) & 1;
-      filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, odd ? bufB : bufA, odd ? bufA : bufB,
-                                                      tw, x, at, tile, tile + 1 < t1, N, Mh, nJ);
+  if constexpr (!BATCHED) {
+    const long t0 = (long)bg * tiles_per_block;
+    const long t1 = min(ntiles, t0 + tiles_per_block);
+    if constexpr (CF::DB) {
+      if (t0 < t1) dma_rows<CF, C>(bufA, x, t0 * CF::TJ * DMU * S, N, th.wave, th.lane);
+      wait_vmcnt<0>();
+      lds_barrier();
+      for (long tile = t0; tile < t1; ++tile) {
+        const bool odd = (tile - t0) & 1;
+        filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, odd ? bufB : bufA, odd ? bufA : bufB,
+                                                        tw, x, x, at, tile, tile + 1, tile + 1 < t1,
+                                                        N, Mh, nJ);
+      }
+    } else {
+      lds_barrier();
+      for (long tile = t0; tile < t1; ++tile)
+        filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, bufA, bufA, tw, x, x, at, tile, tile,
+                                                        false, N, Mh, nJ);
     }
   } else {
+    // the tiles of all members form one range (member-major), split into contiguous block
+    // ranges: a block keeps its taps across the members of its range
+    const long total = ntiles * b.n;
+    const long t0 = (long)bg * tiles_per_block;
+    const long t1 = min(total, t0 + tiles_per_block);
+    if (t0 >= t1) return;
+    long m = t0 / ntiles, tile = t0 - m * ntiles;
+    const C *xc = member_in(x, b, m);
+    C *ac = member_out(at, b, m);
+    if constexpr (CF::DB) {
+      dma_rows<CF, C>(bufA, xc, tile * CF::TJ * DMU * S, N, th.wave, th.lane);
+      wait_vmcnt<0>();
+    }
     lds_barrier();
-    for (long tile = t0; tile < t1; ++tile)
-      filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, bufA, bufA, tw, x, at, tile, false, N,
-                                                      Mh, nJ);
+    for (long f = t0; f < t1; ++f) {
+      const bool odd = (f - t0) & 1, more = f + 1 < t1;
+      long tn = tile + 1, mn = m;
+      if (tn == ntiles) tn = 0, ++mn;
+      const C *xn = mn == m || !more ? xc : member_in(x, b, mn);
+      if constexpr (CF::DB)
+        filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, odd ? bufB : bufA, odd ? bufA : bufB,
+                                                        tw, xc, xn, ac, tile, tn, more, N, Mh, nJ);
+      else
+        filter_tile<CF, S, NMU, NT, DMU, BP, U, NP, CT, C>(th, bufA, bufA, tw, xc, xc, ac, tile, tile,
+                                                        false, N, Mh, nJ);
+      if (!more) break;
+      tile = tn;
+      if (mn != m) {
+        m = mn;
+        xc = xn;
+        ac = member_out(at, b, m);
+      }
+    }
   }
 }
 
@@ -636,15 +689,17 @@ static int launch1(const SoiFilterArgs *a, hipStream_t st)
   typedef FilterCfg<S, NMU, NT, DMU, BP, U, NP, W, CT, C> CF;
   static_assert(CF::LDS_BYTES <= 160 * 1024, "filter tile exceeds LDS");
   if ((long)CF::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
-  auto kern = k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT, C>;
-  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
+  auto kern = a->b.n > 0 ? k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT, C, true>
+                         : k_filter<S, NMU, NT, DMU, BP, U, NP, W, CT, C, false>;
+  // tiles of all members (one transform, or the members of a batched launch)
+  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ * (a->b.n > 0 ? a->b.n : 1);
   // blocks per phase group: enough to fill the CUs, tiles split into contiguous ranges
   long nb = 256L * CF::BPC / CF::NG;
   if (nb < 1) nb = 1;
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * CF::NG), a->b.n > 0 ? a->b.n : 1), dim3(CF::T), 0,
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * CF::NG)), dim3(CF::T), 0,
                      st, (const C *)a->x, (C *)a->at, (const real_t<C> *)a->taps,
                      (const C *)a->post, (const C *)a->twS, a->M, a->Mh, a->nJ, tpb, a->b);
   return (int)hipGetLastError();

@@ -181,6 +181,24 @@ int soi_batch_run(soi_batch_t *b, void *stream);   /* NULL = the batch's own str
 void *soi_batch_stream(soi_batch_t *b);
 void soi_batch_destroy(soi_batch_t *b);
 
+/* ---- test-harness entry points (reference soi.h:282-288, 306, 314-315) --------------------
+ * Host buffers; input kinds 0 (rectangle), 1 (e^{i idx}) and 3 (Dirichlet kernel) with their
+ * closed-form transforms (reference input.c:218-290).  Kinds 2, 4 and 5 need MKL VML/CDFT or
+ * data files: they set SOI_ERR_ARGS, leave the buffer untouched and compute_* return NaN.
+ * reference_output of kind 3 is the reference's value, 1/N of the exact transform.  On one GPU
+ * offset = 0 and localLen = globalLen cover the whole transform. */
+void populate_input(cfft_complex_t *input, size_t localLen, size_t offset, size_t globalLen,
+                    int kind);
+cfft_complex_t reference_output(size_t idx, size_t globalLen, int kind, size_t offset);
+double compute_snr(cfft_complex_t *output, size_t localLen, size_t offset, size_t globalLen,
+                   int kind, soi_desc_t *d);
+double compute_normalized_inf_norm(cfft_complex_t *output, size_t localLen, size_t offset,
+                                   size_t globalLen, int kind);
+double get_cpu_freq(void); /* TSC ticks per second */
+#define SOI_MAX_THREADS 512 /* reference soi.h MAX_THREADS */
+/* per-thread cycle counters of the reference's CPU loops; the GPU path leaves them zero */
+extern unsigned long long load_imbalance_times[SOI_MAX_THREADS];
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("SOI_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["soi_kernels.hip", "soi_filter.hip", "soi_fft.hip"]
-C_SOURCES = ["pfft.c", "compress_host.c"]
+C_SOURCES = ["pfft.c", "compress_host.c", "harness.c"]
 DRIVER = os.path.join(HERE, "soi_test")   # command-line driver (soi_test.c), links the library
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.

@@ -45,6 +45,12 @@ static void clear_err(void)
 }
 const char *soi_last_error(void) { return g_err; }
 int soi_last_status(void) { return g_status; }
+/* for the library's other host files (harness.c): status 0 clears */
+void soi_internal_set_error(int status, const char *msg)
+{
+  if (!status) clear_err();
+  else set_err(status, "%s", msg);
+}
 
 #define HIPCHK(call)                                                                   \
   do {                                                                                 \

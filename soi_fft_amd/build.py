@@ -27,7 +27,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-function",
+CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-function", "-fopenmp",
             "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 
 
@@ -78,7 +78,7 @@ def build(verbose=False, incremental=False):
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs +
-         [f"-L{ROCM}/lib", "-lamdhip64", "-lm"])
+         [f"-L{ROCM}/lib", "-lamdhip64", "-lgomp", "-lm"])
     os.replace(tmp, LIB)
     _run([cc, "-O2", "-std=gnu11", "-Wall", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
           f"-I{ROCM}/include", os.path.join(CSRC, "soi_test.c"), "-o", DRIVER + ".tmp",

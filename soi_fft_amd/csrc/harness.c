@@ -14,7 +14,8 @@
  *
  * On one GPU the whole transform is one "rank": offset and localLen select a window of the
  * global index range exactly as for a reference rank, so a single call with offset 0 and
- * localLen = globalLen covers everything.
+ * localLen = globalLen covers everything.  The element loops are OpenMP-parallel, as the
+ * reference's are (input.c:218-231, 374-424).
  */
 #include <math.h>
 #include <stdint.h>
@@ -80,6 +81,7 @@ void populate_input(cfft_complex_t *input, size_t localLen, size_t offset, size_
                     int kind)
 {
   if (!input || !kind_ok(kind)) return;
+#pragma omp parallel for schedule(static)
   for (size_t j = 0; j < localLen; ++j) {
     const size_t idx = j + offset;
     if (kind == 1) { /* e^{i idx theta} */
@@ -93,17 +95,11 @@ void populate_input(cfft_complex_t *input, size_t localLen, size_t offset, size_
   }
 }
 
-/* Kind 3 returns the reference's value (input.c:284-286), the rectangle of height 1/W; the
- * exact forward DFT of the kind-3 input is N times that. */
-cfft_complex_t reference_output(size_t idx, size_t globalLen, int kind, size_t offset)
+/* expected output of a supported kind (kind_ok checked by the caller) */
+static cfft_complex_t expected(size_t idx, size_t globalLen, int kind)
 {
   cfft_complex_t r;
   put(&r, 0, 0);
-  (void)offset;
-  if (!kind_ok(kind)) {
-    put(&r, NAN, NAN);
-    return r;
-  }
   if (kind == 1) { /* (1 - e^{i N theta}) / (1 - e^{i (theta - 2 pi idx/N)}) */
     const long double a = THETA * globalLen;
     const long double nr = 1 - cosl(a), ni = -sinl(a);
@@ -120,22 +116,40 @@ cfft_complex_t reference_output(size_t idx, size_t globalLen, int kind, size_t o
   return r;
 }
 
+/* Kind 3 returns the reference's value (input.c:284-286), the rectangle of height 1/W; the
+ * exact forward DFT of the kind-3 input is N times that. */
+cfft_complex_t reference_output(size_t idx, size_t globalLen, int kind, size_t offset)
+{
+  (void)offset;
+  if (!kind_ok(kind)) {
+    cfft_complex_t r;
+    put(&r, NAN, NAN);
+    return r;
+  }
+  return expected(idx, globalLen, kind);
+}
+
 static void errors(const cfft_complex_t *output, size_t localLen, size_t offset, size_t N,
                    int kind, double *perr, double *psig, double *merr, double *msig)
 {
-  *perr = *psig = *merr = *msig = 0;
+  double pe = 0, ps = 0, me = 0, ms = 0;
+#pragma omp parallel for schedule(static) reduction(+ : pe, ps) reduction(max : me, ms)
   for (size_t j = 0; j < localLen; ++j) {
-    const cfft_complex_t r = reference_output(j + offset, N, kind, offset);
+    const cfft_complex_t r = expected(j + offset, N, kind);
     double rr, ri, orr, oi;
     get(&r, &rr, &ri);
     get(output + j, &orr, &oi);
     const double dr = orr - rr, di = oi - ri;
     const double e = dr * dr + di * di, s = rr * rr + ri * ri;
-    *perr += e;
-    *psig += s;
-    if (e > *merr) *merr = e;
-    if (s > *msig) *msig = s;
+    pe += e;
+    ps += s;
+    if (e > me) me = e;
+    if (s > ms) ms = s;
   }
+  *perr = pe;
+  *psig = ps;
+  *merr = me;
+  *msig = ms;
 }
 
 double compute_snr(cfft_complex_t *output, size_t localLen, size_t offset, size_t globalLen,

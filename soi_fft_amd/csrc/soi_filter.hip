@@ -107,6 +107,8 @@ template <int n> __device__ __forceinline__ void wait_vmcnt()
 
 // In-place S-point DFT of each of the GR rows of G (row r at G + r*GS, unit stride), Stockham
 // passes of radix 8 then 4 / 2 with compile-time shapes; tw[m] = e^{-2 pi i m/S} in LDS.
+// Butterfly g takes row g % GR: with the odd row stride GS, consecutive lanes then read and
+// write distinct banks.
 template <int S> constexpr int sdft_radix(int pass)
 {
   int rem = S, p = 0, r = 0;
@@ -132,7 +134,7 @@ __device__ __forceinline__ void sdft_passes(C *G, const C *tw, int tid)
     for (int c = 0; c < NB; ++c) {
       const int g = tid + c * T;
       if (TOT % T == 0 || g < TOT) {
-        const int q = g / NBF, j = g % NBF;
+        const int q = g % GR, j = g / GR;  // rows fastest: consecutive lanes on distinct banks
         const C *src = G + q * GS + j;
 #pragma unroll
         for (int k = 0; k < R; ++k) v[c][k] = src[k * NBF];
@@ -143,7 +145,7 @@ __device__ __forceinline__ void sdft_passes(C *G, const C *tw, int tid)
     for (int c = 0; c < NB; ++c) {
       const int g = tid + c * T;
       if (TOT % T == 0 || g < TOT) {
-        const int q = g / NBF, j = g % NBF;
+        const int q = g % GR, j = g / GR;  // rows fastest: consecutive lanes on distinct banks
         const int jm = j % NS;
         if constexpr (NS > 1) {
           constexpr int step = S / (NS * R);

@@ -170,19 +170,38 @@ __device__ __forceinline__ void exchange(C (&v)[E], real_t<C> *lds, int q, int t
   }
 }
 
-// twiddle + DFT of every butterfly of stage s held in registers
+constexpr int bit_len(int x) { return x ? 1 + bit_len(x >> 1) : 0; }
+
+// twiddle + DFT of every butterfly of stage s held in registers.  The R - 1 twiddles w^k of a
+// butterfly come from the log2 R table reads w^(2^b): w^k is the product of the powers of k's
+// set bits (at most three extra roundings for R = 16), so a thread issues a few loads per
+// butterfly instead of R - 1.
 template <int L, int E, int TS, int R, int NS, typename C>
 __device__ __forceinline__ void stage(C (&v)[E], int ts, const C *__restrict__ tw)
 {
   constexpr int NB = E / R;
+  constexpr int NBITS = bit_len(R - 1);
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
     const int j = ts + TS * c;
     if constexpr (NS > 1) {
       const int jm = j % NS;
       constexpr int step = L / (NS * R);
+      C p[NBITS];
 #pragma unroll
-      for (int k = 1; k < R; ++k) v[c * R + k] = cmul(v[c * R + k], tw[k * jm * step]);
+      for (int b = 0; b < NBITS; ++b) p[b] = tw[(jm * step) << b];
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        C w = C{0, 0};
+        bool first = true;
+#pragma unroll
+        for (int b = 0; b < NBITS; ++b)
+          if (k & (1 << b)) {
+            w = first ? p[b] : cmul(w, p[b]);
+            first = false;
+          }
+        v[c * R + k] = cmul(v[c * R + k], w);
+      }
     }
     dft_any<R>(&v[c * R]);
   }

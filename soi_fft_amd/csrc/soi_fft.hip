@@ -271,20 +271,46 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
 
   run_stages<L, E, T, NQP, TS, RS, LY, 0, C>(v, ldsd, q, ts, tw);
 
+  // twiddle w_Mhat^{r2 m1}, exponent below M_hat < 2^32, from the two-level table
+  // (hi[e >> lb1] * lo[e & mask]).  Where the register budget allows (complex64, or one block
+  // of at most 512 threads per CU) it is factored: m1 = out_index(c, 0) + k*L/RL, so the twiddle is
+  // w^{r2 out_index(c, 0)} times w^{r2 k L/RL}, and a thread reads E/RL + RL - 1 table pairs
+  // instead of E.
   constexpr int RL = RS::at(RS::n - 1);
+  constexpr bool FACT = sizeof(real_t<C>) == 4 || (BPC == 1 && T * (L / E) <= 512);
+  const unsigned r2 = (unsigned)(c0 + q), lmask = (1u << lb1) - 1;
+  C tk[RL];
+  if constexpr (FACT) {
 #pragma unroll
-  for (int c = 0; c < E / RL; ++c)
+    for (int k = 1; k < RL; ++k) {
+      const unsigned ex = r2 * (unsigned)(k * (L / RL));
+      tk[k] = cmul(thi[ex >> lb1], tlo[ex & lmask]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < E / RL; ++c) {
+    C tc = C{1, 0};
+    if constexpr (FACT) {
+      const unsigned ex0 = r2 * (unsigned)out_index<L, TS, RS>(ts, c, 0);
+      tc = cmul(thi[ex0 >> lb1], tlo[ex0 & lmask]);
+    }
 #pragma unroll
     for (int k = 0; k < RL; ++k) {
       const int m1 = out_index<L, TS, RS>(ts, c, k);
-      const unsigned ex = (unsigned)(c0 + q) * (unsigned)m1;   // < M_hat < 2^32
-      const C t = cmul(thi[ex >> lb1], tlo[ex & ((1u << lb1) - 1)]);
+      C t;
+      if constexpr (FACT) {
+        t = k ? cmul(tc, tk[k]) : tc;
+      } else {
+        const unsigned ex = r2 * (unsigned)m1;
+        t = cmul(thi[ex >> lb1], tlo[ex & lmask]);
+      }
       if constexpr (TILED)  // 8-row tiles: element (m1, r2) at ((m1/8)*n2 + r2)*8 + m1%8
         tout[(long)s * Mh + (long)(((unsigned)(m1 >> 3) * (unsigned)n2 + (unsigned)(c0 + q)) * 8u +
                                    (unsigned)(m1 & 7))] = cmul(v[c * RL + k], t);
       else
         seg[(unsigned)m1 * (unsigned)n2s] = cmul(v[c * RL + k], t);
     }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------

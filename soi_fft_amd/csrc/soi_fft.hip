@@ -153,7 +153,7 @@ __device__ __forceinline__ void exchange(C (&v)[E], real_t<C> *lds, int q, int t
             lds[LY::at(base + k * NS, ql)] = half ? v[c * R + k].y : v[c * R + k].x;
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (qg == g) {
 #pragma unroll
         for (int c = 0; c < NB2; ++c) {
@@ -165,7 +165,7 @@ __device__ __forceinline__ void exchange(C (&v)[E], real_t<C> *lds, int q, int t
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
 }
@@ -419,14 +419,14 @@ __global__ __launch_bounds__(T * (L / E), fft_minw(BPC, T * (L / E))) void k_fft
           for (int k = 0; k < RL; ++k)
             ldsd[LY::at(out_index<L, TS, RS>(ts, c, k), ql)] = half ? v[c * RL + k].y : v[c * RL + k].x;
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int c = 0; c < EG; ++c) {
         const int e = tid + c * NT;  // e = m2*TG + qq
         const real_t<C> val = ldsd[LY::at(e / TG, e % TG)];
         if (half) w[c].y = val; else w[c].x = val;
       }
-      __syncthreads();
+      lds_barrier();
     }
 #pragma unroll
     for (int c = 0; c < EG; ++c) {

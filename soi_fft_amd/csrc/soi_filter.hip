@@ -184,14 +184,22 @@ __device__ __forceinline__ float2 slice_sum(float2 a, float2 b, int p)
 }
 
 // Stage the XN input values of a tile (contiguous from element g0, wrapping once at N) into
-// dst with direct-to-LDS loads; wave w moves 64-value chunks w, w + NWAVES, ...
+// dst with direct-to-LDS loads; the loading waves take 64-value chunks in turn.  A 10-wave
+// block spreads 3/3/2/2 over the SIMDs (wave w on SIMD w % 4) and its FMA block ends last on
+// SIMDs 0 and 1, so there only the four waves of SIMDs 2 and 3 issue loads; otherwise all do.
 template <typename CF, typename C>
 __device__ __forceinline__ void dma_rows(C *dst, const C *__restrict__ x, long g0, long N,
                                          int wave, int lane)
 {
   constexpr int CH = 64 * CF::EPL;  // values per wave-wide load
+  int slot = wave, nslot = CF::NWAVES;
+  if constexpr (CF::NWAVES == 10) {
+    if ((wave & 3) < 2) return;
+    slot = (wave >> 2) * 2 + (wave & 3) - 2;
+    nslot = 4;
+  }
 #pragma unroll
-  for (int c = wave; c < CF::NDMA; c += CF::NWAVES) {
+  for (int c = slot; c < CF::NDMA; c += nslot) {
     long gi = g0 + c * CH + lane * CF::EPL;  // N and g0 are multiples of EPL: no pair straddles N
     if (gi >= N) gi -= N;
     __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * CH), 16,

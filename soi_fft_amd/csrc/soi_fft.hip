@@ -110,7 +110,8 @@ template <int... Rs> struct Radices {
 //   QFast:    sequence index fastest, (pos + pos/16)*TG + q -- suits waves that span the T
 //             sequences at a few positions (the column pass, whose lanes run over q first)
 //   QFast8:   the same with one pad slot per 8 positions: the 8-wide 2560-point columns, whose
-//             radix-10 stage writes positions 10 apart (complex64 pass 1 1.12 -> 1.08 ms)
+//             radix-10 stage writes positions 10 apart (complex64: 1.63 -> 1.40 ms for that
+//             variant; the default 4-wide one keeps QFast)
 //   SeqMajor: one padded run per sequence, q*LP + pos + pos/16 with LP = 2 (mod 16) -- suits
 //             waves that hold consecutive positions of one sequence (the row pass); the LP offset
 //             keeps the T-sequence transpose reads of the row pass on distinct banks

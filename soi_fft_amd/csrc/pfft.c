@@ -535,15 +535,18 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       pl->fa.post = pl->post32;
       pl->fa.twS = pl->twS32;
       pl->fa.f32 = 1;
-      /* SOI_FILTER_MFMA=1: the filter contraction on MFMA with split-bf16 operands where its
-       * shape allows.  Off by default: at S = 32 the per-lane tap operands take 144 of the
-       * 256 registers of its 512-thread blocks and it runs slower than the FP32 VALU form. */
+      /* SOI_FILTER_MFMA=1: the filter contraction on MFMA with split-bf16 operands, =2: with
+       * FP32 operands (v_mfma_f32_16x16x4_f32), where the shape allows.  Off by default: both
+       * measured slower than the FP32 VALU form. */
       const char *em = getenv("SOI_FILTER_MFMA");
-      if (em && atoi(em) != 0 &&
-          soik_filter_mfma_supported((int)S, d->n_mu, d->d_mu, pl->fa.ct, B_pad)) {
-        HIPCHK(hipMalloc(&pl->wt, soik_filter_mfma_table_bytes((int)S, B_pad)));
-        KCHK(soik_filter_mfma_table((const double *)pl->taps, pl->wt, (int)S, B_pad, pl->stream));
+      const int mode = em ? atoi(em) : 0;
+      if ((mode == 1 || mode == 2) &&
+          soik_filter_mfma_supported((int)S, d->n_mu, d->d_mu, pl->fa.ct, B_pad, mode)) {
+        HIPCHK(hipMalloc(&pl->wt, soik_filter_mfma_table_bytes((int)S, B_pad, mode)));
+        KCHK(soik_filter_mfma_table((const double *)pl->taps, pl->wt, (int)S, B_pad, mode,
+                                    pl->stream));
         pl->fa.wt = pl->wt;
+        pl->fa.mfma = mode;
       }
       if (pl->fft_single) {
         HIPCHK(hipMalloc((void **)&pl->tw1_32, (size_t)pl->Mh * sizeof(soi_c32_t)));

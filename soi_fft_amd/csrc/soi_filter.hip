@@ -659,23 +659,214 @@ template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
   return (int)hipGetLastError();
 }
 
-// The MFMA form applies to the complex64 real-tap (5, 4) filter with S in {4, 8, 16, 32} and at
-// most 88 padded taps.
-extern "C" int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD)
+// ---------------------------------------------------------------------------------------------
+// Complex64 real-tap filter on MFMA with FP32 operands (v_mfma_f32_16x16x4_f32: exact FP32
+// products, FP32 accumulation).  The GEMM of each lane i is the one of the split-bf16 form,
+//   C[2m + part][theta + 5D] = sum_k x_i[4 J0 + 12 m + k].part * t_i[theta][k - 4D],
+// with K = 80 >= BPAD + 8 covered by 20 four-deep steps.  The A operand of step ks in lane l is
+// row (m, part) = ((l % 16) >> 1, l & 1), k = l / 16, read from an LDS copy of the tile rows
+// transposed to one FP32 run per (lane i, part); the B operands (20 per lane i) stay in
+// registers.  Tile pipeline, post phase, S-point DFT and stores as in k_filter_mfma.
+// ---------------------------------------------------------------------------------------------
+template <int S> struct Mfma32Cfg {
+  static constexpr int EPL = 2;                 // complex64 values per 16-byte direct-to-LDS load
+  static constexpr int TJ = 24;                 // row groups per tile: 8 MFMA rows x 3 shifts
+  static constexpr int NK = 20;                 // four-deep K steps
+  static constexpr int K = 4 * NK;
+  static constexpr int NRG = (12 * 7 + K + 7) / 8;  // 8-row groups staged per tile
+  static constexpr int ROWS = NRG * 8;
+  static constexpr int RP = ((ROWS + 63) / 64) * 64 + 32;  // run pitch: the two parts 32 banks apart
+  static constexpr int LPW = S >= 32 ? 4 : S < 2 ? S : 2;  // lanes per wave (<= 8 waves)
+  static constexpr int NWAVES = S / LPW;
+  static constexpr int T = NWAVES * 64;
+  static constexpr int XN = (ROWS * S + 64 * EPL - 1) / (64 * EPL) * (64 * EPL);  // raw values
+  static constexpr int NDMA = XN / (64 * EPL);  // wave-wide 16-byte loads per tile
+  static constexpr int GR = TJ * 5;             // output rows R per tile
+  static constexpr int GS = S + 1;
+  static constexpr int NSC = (GR * S + T - 1) / T;
+  static constexpr int NSC_MIN = (GR * S) / T;
+};
+
+template <int S>
+__global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filter_mfma32(
+    const float2 *__restrict__ x, float2 *__restrict__ at, const float *__restrict__ wt,
+    const float2 *__restrict__ post, const float2 *__restrict__ twS, long M, long Mh, long nJ,
+    int tiles_per_block, SoiBatch b)
 {
-  return !ct && n_mu == 5 && d_mu == 4 && BPAD <= 88 && (S == 4 || S == 8 || S == 16 || S == 32);
+  typedef Mfma32Cfg<S> CF;
+  x = batch_ptr(x, b.x);
+  at = batch_ptr(at, b.at);
+  constexpr int LPW = CF::LPW, NK = CF::NK, RP = CF::RP;
+  __shared__ __align__(16) float2 raw[CF::XN];
+  __shared__ __align__(16) float As[S * 2 * RP];
+  __shared__ __align__(16) float2 G[CF::GR * CF::GS];
+  __shared__ __align__(16) float2 tw[S];
+  __shared__ __align__(16) float2 pfl[5 * S];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int n = lane & 15, g = lane >> 4;
+
+  float wb[LPW][NK];  // B operands of this wave's lanes
+#pragma unroll
+  for (int li = 0; li < LPW; ++li)
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((wave * LPW + li) * NK + ks) * 64 + lane];
+  for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
+  for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
+
+  const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
+  const long t0 = (long)blockIdx.x * tiles_per_block;
+  const long t1 = min(ntiles, t0 + tiles_per_block);
+  const long N = M * S;
+  if (t0 >= t1) return;
+
+  // raw rows of a tile -> As[(i*2 + part)*RP + row]
+  auto transpose_tile = [&]() {
+    for (int task = tid; task < CF::NRG * S; task += CF::T) {
+      const int i = task % S, rg = task / S;
+      float4 re[2], im[2];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const float2 v = raw[(rg * 8 + r) * S + i];
+        (r < 4 ? re[0] : re[1])[r & 3] = v.x;
+        (r < 4 ? im[0] : im[1])[r & 3] = v.y;
+      }
+      float4 *dre = reinterpret_cast<float4 *>(As + (i * 2 + 0) * RP + rg * 8);
+      float4 *dim = reinterpret_cast<float4 *>(As + (i * 2 + 1) * RP + rg * 8);
+      dre[0] = re[0], dre[1] = re[1], dim[0] = im[0], dim[1] = im[1];
+    }
+  };
+
+  dma_rows<CF, float2>(raw, x, t0 * CF::TJ * 4 * S, N, wave, lane);
+  wait_vmcnt<0>();
+  lds_barrier();
+  transpose_tile();
+  lds_barrier();
+  if (t0 + 1 < t1) dma_rows<CF, float2>(raw, x, (t0 + 1) * CF::TJ * 4 * S, N, wave, lane);
+
+  const int am = n >> 1, apart = n & 1;
+  for (long tile = t0; tile < t1; ++tile) {
+    const bool more = tile + 1 < t1;
+
+    // ---- MFMA: one accumulation chain per lane i, all on v_mfma_f32_16x16x4_f32
+    f32x4 acc[LPW];
+#pragma unroll
+    for (int li = 0; li < LPW; ++li) acc[li] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        const int i = wave * LPW + li;
+        const float a = As[(i * 2 + apart) * RP + 12 * am + 4 * ks + g];
+        acc[li] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wb[li][ks], acc[li], 0, 0, 0);
+      }
+    }
+    lds_barrier();  // every wave is done reading As
+
+    // ---- post phase into G: accumulator j of this lane is part j & 1 of row m = 2g + (j >> 1)
+    if (n < 15) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        const int i = wave * LPW + li;
+        const float2 pf = pfl[(n % 5) * S + i];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          G[(15 * (2 * g + h) + n) * CF::GS + i] = cmul(float2{acc[li][2 * h], acc[li][2 * h + 1]}, pf);
+      }
+    }
+    if (more) {
+      asm volatile("" ::: "memory");
+      if (tile == t0) wait_vmcnt<0>();
+      else wait_vmcnt<CF::NSC_MIN>();
+      lds_barrier();
+      transpose_tile();
+    }
+    lds_barrier();  // publishes G and the next tile's rows; raw is free again
+    if (tile + 2 < t1) dma_rows<CF, float2>(raw, x, (tile + 2) * CF::TJ * 4 * S, N, wave, lane);
+
+    // ---- S-point DFT of each row
+    sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, float2>(G, tw, tid);
+
+    // ---- store segment-major: R = 5 J0 + r (runs of GR consecutive R per segment)
+    const long R0 = tile * CF::TJ * 5;
+#pragma unroll
+    for (int c = 0; c < CF::NSC; ++c) {
+      const int e = tid + c * CF::T;
+      if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
+        const int s = e / CF::GR, r = e % CF::GR;
+        if (tile * CF::TJ + r / 5 < nJ) store_nt(at + (long)s * Mh + R0 + r, G[r * CF::GS + s]);
+      }
+    }
+  }
 }
 
-extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD)
+// FP32 B operand table: wt[(i*NK + ks)*64 + lane] = t_i[theta][k - 4D] for k = 4 ks + lane/16
+// and n = lane % 16 = theta + 5D (zero for n = 15 or a tap index outside [0, BPAD)), rounded
+// once from the FP64 real-tap table taps[(theta*S + i)*BPAD + b].
+__global__ void k_filter_mfma32_table(const double *__restrict__ taps, float *__restrict__ wt, int S,
+                                      int BPAD, int NK)
+{
+  const long total = (long)S * NK * 64;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(e % 64), ks = (int)(e / 64 % NK), i = (int)(e / (64L * NK));
+    const int n = lane % 16, k = 4 * ks + lane / 16;
+    const int theta = n % 5, D = n / 5, bb = k - 4 * D;
+    const bool on = n < 15 && bb >= 0 && bb < BPAD;
+    wt[e] = on ? (float)taps[((long)theta * S + i) * BPAD + bb] : 0.0f;
+  }
+}
+
+template <int S> static int launch_mfma32(const SoiFilterArgs *a, hipStream_t st)
+{
+  typedef Mfma32Cfg<S> CF;
+  if ((long)CF::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
+  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
+  long nb = 256;
+  if (nb > ntiles) nb = ntiles;
+  const int tpb = (int)((ntiles + nb - 1) / nb);
+  nb = (ntiles + tpb - 1) / tpb;
+  hipLaunchKernelGGL(k_filter_mfma32<S>, dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1), dim3(CF::T), 0,
+                     st, (const float2 *)a->x, (float2 *)a->at, (const float *)a->wt,
+                     (const float2 *)a->post, (const float2 *)a->twS, a->M, a->Mh, a->nJ, tpb,
+                     a->b);
+  return (int)hipGetLastError();
+}
+
+static int filter_mfma32(const SoiFilterArgs *a, hipStream_t st)
+{
+  switch (a->S) {
+    case 4: return launch_mfma32<4>(a, st);
+    case 8: return launch_mfma32<8>(a, st);
+    case 16: return launch_mfma32<16>(a, st);
+    case 32: return launch_mfma32<32>(a, st);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
+// The MFMA forms apply to the complex64 real-tap (5, 4) filter with S in {4, 8, 16, 32} and at
+// most 88 (split-bf16, mode 1) or 72 (FP32, mode 2) padded taps.
+extern "C" int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD, int mode)
+{
+  const int maxb = mode == 2 ? Mfma32Cfg<4>::K - 8 : 88;
+  return (mode == 1 || mode == 2) && !ct && n_mu == 5 && d_mu == 4 && BPAD <= maxb &&
+         (S == 4 || S == 8 || S == 16 || S == 32);
+}
+
+extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD, int mode)
 {
   (void)BPAD;
+  if (mode == 2) return (size_t)S * Mfma32Cfg<4>::NK * 64 * sizeof(float);
   return (size_t)S * MfmaCfg<4>::NCH * 3 * 64 * 16;
 }
 
-extern "C" int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, hipStream_t st)
+extern "C" int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, int mode,
+                                      hipStream_t st)
 {
-  hipLaunchKernelGGL(k_filter_mfma_table, dim3(64), dim3(256), 0, st, taps, (unsigned short *)wt, S,
-                     BPAD, MfmaCfg<4>::NCH);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_filter_mfma32_table, dim3(64), dim3(256), 0, st, taps, (float *)wt, S,
+                       BPAD, Mfma32Cfg<4>::NK);
+  else
+    hipLaunchKernelGGL(k_filter_mfma_table, dim3(64), dim3(256), 0, st, taps, (unsigned short *)wt,
+                       S, BPAD, MfmaCfg<4>::NCH);
   return (int)hipGetLastError();
 }
 
@@ -800,7 +991,7 @@ extern "C" size_t soik_filter_lds_bytes(const SoiFilterArgs *a)
 
 extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
 {
-  if (a->f32 && a->wt) return filter_mfma(a, st);
+  if (a->f32 && a->wt) return a->mfma == 2 ? filter_mfma32(a, st) : filter_mfma(a, st);
   if (!a->ct) {
     RT_CASE(4, 32, 16) RT_CASE(4, 36, 16) RT_CASE(4, 40, 16)
     RT_CASE(8, 32, 8) RT_CASE(8, 36, 8) RT_CASE(8, 40, 8)

@@ -35,9 +35,10 @@ typedef struct SoiFilterArgs {
   const void *taps;   /* [n_mu][S][NP*BP], real (ct = 0) or complex (ct = 1), double or float */
   const void *post;   /* [n_mu][S] post phase, complex of the same precision */
   const void *twS;    /* S-point twiddles, complex of the same precision */
-  const void *wt;     /* complex64 only: bf16 tap pieces of the MFMA form (NULL: VALU form) */
+  const void *wt;     /* complex64 only: tap operands of an MFMA form (NULL: VALU form) */
   long M, Mh, nJ;
   int S, n_mu, d_mu, BP, NP, NT, U, W, ct, f32;
+  int mfma;           /* MFMA form of wt: 1 = split-bf16 pieces, 2 = FP32 */
   SoiBatch b;         /* batched launch tables (x -> b.x[t], at -> b.at[t]) */
 } SoiFilterArgs;
 
@@ -49,9 +50,10 @@ int soik_filter_shape(int S, int n_mu, int d_mu, int B, int ct, int *NP, int *BP
 size_t soik_filter_lds_bytes(const SoiFilterArgs *a);
 int soik_make_twiddles(soi_c64_t *tw, long n, long L, long mult, hipStream_t st);
 int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st);
-int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD);
-size_t soik_filter_mfma_table_bytes(int S, int BPAD);
-int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, hipStream_t st);
+int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD, int mode);
+size_t soik_filter_mfma_table_bytes(int S, int BPAD, int mode);
+int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, int mode,
+                           hipStream_t st);
 int soik_fft_single(const soi_c64_t *at, soi_c64_t *X, const soi_c64_t *tw,
                     const soi_c64_t *winv, long L, long M, int S, int count, SoiBatch b, hipStream_t st);
 int soik_fft_col(soi_c64_t *data, const soi_c64_t *tw1, const soi_c64_t *thi,

@@ -132,8 +132,9 @@ def bench_fp32(args, soi, torch, x64, N, P, fp64_gflops):
             "speedup_vs_fp64_form": round(gflops / fp64_gflops, 3),
             "stages": {"filter_sdft_ms": round(acc[0], 4), "fft_pass1_ms": round(acc[2], 4),
                        "fft_pass2_demod_ms": round(acc[3], 4)},
-            "form": "complex64 data; FP32 filter contraction (VALU) and FFTs; FP64 window, "
-                    "twiddle and W_inv generation; W_inv applied in FP64"}
+            "form": "complex64 data; FP32 filter contraction on MFMA (v_mfma_f32_16x16x4_f32, "
+                    "exact FP32 products) and FP32 FFTs; FP64 window, twiddle and W_inv "
+                    "generation; W_inv applied in FP64"}
 
 
 def bench_batch(args, soi, torch, prec):

@@ -535,11 +535,11 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       pl->fa.post = pl->post32;
       pl->fa.twS = pl->twS32;
       pl->fa.f32 = 1;
-      /* SOI_FILTER_MFMA=1: the filter contraction on MFMA with split-bf16 operands, =2: with
-       * FP32 operands (v_mfma_f32_16x16x4_f32), where the shape allows.  Off by default: both
-       * measured slower than the FP32 VALU form. */
+      /* The filter contraction on MFMA with FP32 operands (v_mfma_f32_16x16x4_f32) where the
+       * shape allows: the default, faster than the FP32 VALU form.  SOI_FILTER_MFMA=0 selects
+       * the VALU form, =1 the split-bf16 MFMA form (slower than both). */
       const char *em = getenv("SOI_FILTER_MFMA");
-      const int mode = em ? atoi(em) : 0;
+      const int mode = em ? atoi(em) : 2;
       if ((mode == 1 || mode == 2) &&
           soik_filter_mfma_supported((int)S, d->n_mu, d->d_mu, pl->fa.ct, B_pad, mode)) {
         HIPCHK(hipMalloc(&pl->wt, soik_filter_mfma_table_bytes((int)S, B_pad, mode)));

@@ -664,23 +664,25 @@ template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
 // products, FP32 accumulation).  The GEMM of each lane i is the one of the split-bf16 form,
 //   C[2m + part][theta + 5D] = sum_k x_i[4 J0 + 12 m + k].part * t_i[theta][k - 4D],
 // with K = 80 >= BPAD + 8 covered by 20 four-deep steps.  The A operand of step ks in lane l is
-// row (m, part) = ((l % 16) >> 1, l & 1), k = l / 16, read from an LDS copy of the tile rows
-// transposed to one FP32 run per (lane i, part); the B operands (20 per lane i) stay in
-// registers.  Tile pipeline, post phase, S-point DFT and stores as in k_filter_mfma.
+// row (m, part) = ((l % 16) >> 1, l & 1), k = l / 16, read straight from the tile rows as the
+// direct-to-LDS loads left them: 1 KB chunks with a 16-byte gap after each, so the 16 (m, part)
+// rows of one read fall on distinct banks.  Two row buffers alternate: the loads of tile t + 2
+// fill the buffer of tile t once its MFMA block is done.  The B operands (20 per lane i) stay
+// in registers.  Post phase, S-point DFT and stores as in k_filter_mfma.
 // ---------------------------------------------------------------------------------------------
 template <int S> struct Mfma32Cfg {
   static constexpr int EPL = 2;                 // complex64 values per 16-byte direct-to-LDS load
   static constexpr int TJ = 24;                 // row groups per tile: 8 MFMA rows x 3 shifts
   static constexpr int NK = 20;                 // four-deep K steps
   static constexpr int K = 4 * NK;
-  static constexpr int NRG = (12 * 7 + K + 7) / 8;  // 8-row groups staged per tile
-  static constexpr int ROWS = NRG * 8;
-  static constexpr int RP = ((ROWS + 63) / 64) * 64 + 32;  // run pitch: the two parts 32 banks apart
+  static constexpr int ROWS = 12 * 7 + K;       // input rows read per tile
   static constexpr int LPW = S >= 32 ? 4 : S < 2 ? S : 2;  // lanes per wave (<= 8 waves)
   static constexpr int NWAVES = S / LPW;
   static constexpr int T = NWAVES * 64;
-  static constexpr int XN = (ROWS * S + 64 * EPL - 1) / (64 * EPL) * (64 * EPL);  // raw values
-  static constexpr int NDMA = XN / (64 * EPL);  // wave-wide 16-byte loads per tile
+  static constexpr int CH = 64 * EPL;           // values per wave-wide load (one chunk)
+  static constexpr int CP = CH + 2;             // chunk pitch in LDS: a 16-byte gap
+  static constexpr int NDMA = (ROWS * S + CH - 1) / CH;  // chunks per tile
+  static constexpr int RAWN = NDMA * CP;
   static constexpr int GR = TJ * 5;             // output rows R per tile
   static constexpr int GS = S + 1;
   static constexpr int NSC = (GR * S + T - 1) / T;
@@ -696,9 +698,9 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
   typedef Mfma32Cfg<S> CF;
   x = batch_ptr(x, b.x);
   at = batch_ptr(at, b.at);
-  constexpr int LPW = CF::LPW, NK = CF::NK, RP = CF::RP;
-  __shared__ __align__(16) float2 raw[CF::XN];
-  __shared__ __align__(16) float As[S * 2 * RP];
+  constexpr int LPW = CF::LPW, NK = CF::NK, CH = CF::CH, CP = CF::CP;
+  __shared__ __align__(16) float2 raw0[CF::RAWN];
+  __shared__ __align__(16) float2 raw1[CF::RAWN];
   __shared__ __align__(16) float2 G[CF::GR * CF::GS];
   __shared__ __align__(16) float2 tw[S];
   __shared__ __align__(16) float2 pfl[5 * S];
@@ -719,33 +721,26 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
   const long N = M * S;
   if (t0 >= t1) return;
 
-  // raw rows of a tile -> As[(i*2 + part)*RP + row]
-  auto transpose_tile = [&]() {
-    for (int task = tid; task < CF::NRG * S; task += CF::T) {
-      const int i = task % S, rg = task / S;
-      float4 re[2], im[2];
+  // the NDMA chunks of a tile's rows (contiguous from element g0, wrapping once at N)
+  auto dma = [&](float2 *dst, long g0) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const float2 v = raw[(rg * 8 + r) * S + i];
-        (r < 4 ? re[0] : re[1])[r & 3] = v.x;
-        (r < 4 ? im[0] : im[1])[r & 3] = v.y;
-      }
-      float4 *dre = reinterpret_cast<float4 *>(As + (i * 2 + 0) * RP + rg * 8);
-      float4 *dim = reinterpret_cast<float4 *>(As + (i * 2 + 1) * RP + rg * 8);
-      dre[0] = re[0], dre[1] = re[1], dim[0] = im[0], dim[1] = im[1];
+    for (int c = wave; c < CF::NDMA; c += CF::NWAVES) {
+      long gi = g0 + c * CH + lane * CF::EPL;  // N and g0 are multiples of EPL
+      if (gi >= N) gi -= N;
+      __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * CP), 16, 0, 0);
     }
   };
 
-  dma_rows<CF, float2>(raw, x, t0 * CF::TJ * 4 * S, N, wave, lane);
+  dma(raw0, t0 * CF::TJ * 4 * S);
   wait_vmcnt<0>();
   lds_barrier();
-  transpose_tile();
-  lds_barrier();
-  if (t0 + 1 < t1) dma_rows<CF, float2>(raw, x, (t0 + 1) * CF::TJ * 4 * S, N, wave, lane);
+  if (t0 + 1 < t1) dma(raw1, (t0 + 1) * CF::TJ * 4 * S);
 
   const int am = n >> 1, apart = n & 1;
   for (long tile = t0; tile < t1; ++tile) {
     const bool more = tile + 1 < t1;
+    float2 *cur = ((tile - t0) & 1) ? raw1 : raw0;
+    const float *curf = reinterpret_cast<const float *>(cur);
 
     // ---- MFMA: one accumulation chain per lane i, all on v_mfma_f32_16x16x4_f32
     f32x4 acc[LPW];
@@ -755,12 +750,12 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
     for (int ks = 0; ks < NK; ++ks) {
 #pragma unroll
       for (int li = 0; li < LPW; ++li) {
-        const int i = wave * LPW + li;
-        const float a = As[(i * 2 + apart) * RP + 12 * am + 4 * ks + g];
+        const int e = (12 * am + 4 * ks + g) * S + wave * LPW + li;  // row * S + lane i
+        const float a = curf[2 * ((e / CH) * CP + e % CH) + apart];
         acc[li] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wb[li][ks], acc[li], 0, 0, 0);
       }
     }
-    lds_barrier();  // every wave is done reading As
+    lds_barrier();  // every wave is done reading cur (and the previous tile's G)
 
     // ---- post phase into G: accumulator j of this lane is part j & 1 of row m = 2g + (j >> 1)
     if (n < 15) {
@@ -774,14 +769,14 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
       }
     }
     if (more) {
+      // the next tile's rows are complete once at most this wave's stores of the previous tile
+      // are outstanding (they were issued after the rows; none before the first tile)
       asm volatile("" ::: "memory");
       if (tile == t0) wait_vmcnt<0>();
       else wait_vmcnt<CF::NSC_MIN>();
-      lds_barrier();
-      transpose_tile();
     }
-    lds_barrier();  // publishes G and the next tile's rows; raw is free again
-    if (tile + 2 < t1) dma_rows<CF, float2>(raw, x, (tile + 2) * CF::TJ * 4 * S, N, wave, lane);
+    lds_barrier();  // publishes G and the next tile's rows; cur is free
+    if (tile + 2 < t1) dma(cur, (tile + 2) * CF::TJ * 4 * S);
 
     // ---- S-point DFT of each row
     sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, float2>(G, tw, tid);
@@ -818,7 +813,7 @@ __global__ void k_filter_mfma32_table(const double *__restrict__ taps, float *__
 template <int S> static int launch_mfma32(const SoiFilterArgs *a, hipStream_t st)
 {
   typedef Mfma32Cfg<S> CF;
-  if ((long)CF::XN > a->M * S) return (int)hipErrorInvalidValue;  // tile window wraps at most once
+  if ((long)CF::NDMA * CF::CH > a->M * S) return (int)hipErrorInvalidValue;  // wraps at most once
   const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
   long nb = 256;
   if (nb > ntiles) nb = ntiles;

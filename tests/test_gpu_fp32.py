@@ -126,13 +126,14 @@ def test_fp32_fft_variants_agree(soi, orc, torch_cuda, monkeypatch, tiled):
 MFMA_SHAPES = [(2 ** 13, 4), (2 ** 16, 8), (2 ** 18, 16), (2 ** 18, 32), (2 ** 20, 8)]
 
 
-@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("mode", ["0", "1", "2"])
 @pytest.mark.parametrize("N,P", MFMA_SHAPES)
 def test_fp32_mfma_filter_matches_fp64_oracle(soi, orc, torch_cuda, monkeypatch, N, P, mode):
-    """The MFMA filter forms for S = 4 .. 32 lanes, one and several tiles per block, against the
-    FP64 oracle at the FP32 bound: SOI_FILTER_MFMA=1 splits every operand into hi + mid + lo
-    bf16 pieces (six piece products), =2 multiplies FP32 operands exactly
-    (v_mfma_f32_16x16x4_f32); both accumulate in FP32."""
+    """The three forms of the complex64 filter contraction for S = 4 .. 32 lanes, one and
+    several tiles per block, against the FP64 oracle at the FP32 bound: SOI_FILTER_MFMA=0 on
+    VALU, =1 on MFMA with every operand split into hi + mid + lo bf16 pieces (six piece
+    products), =2 (the default) on MFMA with exact FP32 products (v_mfma_f32_16x16x4_f32); all
+    accumulate in FP32."""
     monkeypatch.setenv("SOI_FILTER_MFMA", mode)
     print(f"case N={N} P={P} mfma filter mode {mode}", flush=True)
     x32 = orc.uniform_input(N, 303).astype(np.complex64)

@@ -676,7 +676,7 @@ template <int S> struct Mfma32Cfg {
   static constexpr int NK = 20;                 // four-deep K steps
   static constexpr int K = 4 * NK;
   static constexpr int ROWS = 12 * 7 + K;       // input rows read per tile
-  static constexpr int LPW = S >= 32 ? 4 : S < 2 ? S : 2;  // lanes per wave (<= 8 waves)
+  static constexpr int LPW = S < 2 ? S : 2;    // lanes per wave (16 waves at S = 32)
   static constexpr int NWAVES = S / LPW;
   static constexpr int T = NWAVES * 64;
   static constexpr int CH = 64 * EPL;           // values per wave-wide load (one chunk)

@@ -1,7 +1,8 @@
-"""ctypes binding of the CPU oracle (oracle/soi_oracle.c).
+"""ctypes binding of the CPU oracle (oracle/soi_oracle.c) and of the CPU performance port
+(oracle/soi_cpu_port.c, the CPU baseline bench.py times).
 
 TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
-this module as the checker.  The product package (soi_fft_amd) never imports it.
+this module as the checker and the baseline.  The product package (soi_fft_amd) never imports it.
 """
 import ctypes
 import os
@@ -61,6 +62,11 @@ def lib():
         _lib.orc_uniform_input.argtypes = [ctypes.c_long, ctypes.c_uint64, dp]
         _lib.orc_num_threads.restype = ctypes.c_int
         _lib.orc_set_window_mode.argtypes = [ctypes.c_int]
+        _lib.port_create.argtypes = [pp]
+        _lib.port_create.restype = ctypes.c_void_p
+        _lib.port_compute.argtypes = [ctypes.c_void_p, dp]
+        _lib.port_destroy.argtypes = [ctypes.c_void_p]
+        _lib.port_threads.restype = ctypes.c_int
     return _lib
 
 
@@ -186,3 +192,32 @@ def uniform_input(N, seed):
     a = np.zeros(2 * N)
     lib().orc_uniform_input(N, seed, _dp(a))
     return a.view(np.complex128)
+
+
+class CpuPort:
+    """The CPU performance port of compute_soi (soi_cpu_port.c): plan once, transform in place.
+    A baseline to time beside the GPU, not the checker."""
+
+    def __init__(self, p):
+        self._p = p
+        self.h = lib().port_create(ctypes.byref(p))
+        if not self.h:
+            raise ValueError("port_create: unsupported shape")
+
+    def compute(self, x):
+        """x: complex128 array of N values, transformed in place."""
+        assert x.dtype == np.complex128 and x.flags["C_CONTIGUOUS"] and x.size == self._p.N
+        lib().port_compute(self.h, x.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return x
+
+    def close(self):
+        if self.h:
+            lib().port_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def port_threads():
+    return lib().port_threads()

@@ -26,6 +26,11 @@ SOI_WINDOW_REFERENCE = 1
 SOI_WINDOW_EXACT = 2
 
 
+# status codes (soi.h)
+SOI_OK, SOI_ERR_ARGS, SOI_ERR_TOO_SMALL, SOI_ERR_UNSUPPORTED_MU = 0, 1, 2, 3
+SOI_ERR_ALLOC, SOI_ERR_HIP, SOI_ERR_STATE = 4, 5, 6
+
+
 class SoiError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"soi error {status}: {msg}")

@@ -105,7 +105,13 @@ typedef struct {
   soi_c64_t *tw1, *tw2, *thi, *tlo;
   /* buffers */
   soi_c64_t *io;        /* staging for host-pointer inputs */
+  /* the device buffers the kernels use, kept here as well as in the descriptor fields: a
+   * reference-style caller may null or overwrite d->alpha_tilde, d->w and d->W_inv between
+   * calls (test.c:406-420), and the plan still reaches and frees what it owns */
+  void *at;             /* alpha_tilde: the caller's preallocated buffer or owns_at */
   int owns_at;
+  soi_c64_t *wtab, *winv;
+  int *delta;
   /* vlc */
   int *d_exps, *d_emi, *d_nwords;
   int *h_exps, *seg_max, *send_counts, total_max;
@@ -378,6 +384,10 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->tlo);
   (void)hipFree(pl->io);
   (void)hipFree(pl->tbuf);
+  if (pl->owns_at) (void)hipFree(pl->at);
+  (void)hipFree(pl->wtab);
+  (void)hipFree(pl->winv);
+  (void)hipFree(pl->delta);
   (void)hipFree(pl->d_exps);
   (void)hipFree(pl->d_emi);
   (void)hipFree(pl->d_nwords);
@@ -477,17 +487,35 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     size_t ntap = (size_t)d->n_mu * S * B_pad;
     HIPCHK(hipMalloc((void **)&pl->taps, ntap * (pl->fa.ct ? 2 : 1) * sizeof(double)));
     HIPCHK(hipMalloc((void **)&pl->post, (size_t)d->n_mu * S * sizeof(soi_c64_t)));
-    HIPCHK(hipMalloc((void **)&d->w, (size_t)d->n_mu * d->B * S * sizeof(soi_c64_t)));
-    HIPCHK(hipMalloc((void **)&d->W_inv, (size_t)M * sizeof(soi_c64_t)));
+    HIPCHK(hipMalloc((void **)&pl->wtab, (size_t)d->n_mu * d->B * S * sizeof(soi_c64_t)));
+    HIPCHK(hipMalloc((void **)&pl->winv, (size_t)M * sizeof(soi_c64_t)));
     HIPCHK(hipMalloc((void **)&pl->twS, (size_t)S * sizeof(soi_c64_t)));
     d->w_dup = pl->taps;
     const size_t esz = pl->f32 ? sizeof(soi_c32_t) : sizeof(soi_c64_t);
-    if (!d->alpha_tilde) {
-      hipError_t e = hipMalloc((void **)&d->alpha_tilde, (size_t)S * pl->Mh * esz);
+    const size_t at_bytes = (size_t)S * pl->Mh * esz;
+    if (d->alpha_tilde) {
+      /* a preallocated alpha_tilde holds all S segments ([S][M_hat], S*M_hat values): the
+       * reference's per-rank M_hat*k (pfft.c:161-163) is P times too small here */
+      void *base = NULL;
+      size_t size = 0;
+      if (hipMemGetAddressRange(&base, &size, d->alpha_tilde) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        set_err(SOI_ERR_ARGS, "preallocated alpha_tilde is not device memory");
+        goto fail_quiet;
+      }
+      const size_t avail = (size_t)((char *)base + size - (char *)d->alpha_tilde);
+      if (avail < at_bytes) {
+        set_err(SOI_ERR_ARGS, "preallocated alpha_tilde holds %zu bytes; S*M_hat values need "
+                "%zu (S=%ld, M_hat=%ld)", avail, at_bytes, S, pl->Mh);
+        goto fail_quiet;
+      }
+      pl->at = d->alpha_tilde;
+    } else {
+      hipError_t e = hipMalloc(&pl->at, at_bytes);
       if (e != hipSuccess) {
-        d->alpha_tilde = NULL;
-        set_err(SOI_ERR_ALLOC, "Failed to allocate d->alpha_tilde (%zu bytes): %s",
-                (size_t)S * pl->Mh * esz, hipGetErrorString(e));
+        pl->at = NULL;
+        set_err(SOI_ERR_ALLOC, "Failed to allocate d->alpha_tilde (%zu bytes): %s", at_bytes,
+                hipGetErrorString(e));
         goto fail_quiet;
       }
       pl->owns_at = 1;
@@ -501,7 +529,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
         goto fail_quiet;
       }
     }
-    KCHK(soik_make_tables(pl->taps, pl->post, (soi_c64_t *)d->w, (soi_c64_t *)d->W_inv, (int)S,
+    KCHK(soik_make_tables(pl->taps, pl->post, pl->wtab, pl->winv, (int)S,
                           d->n_mu, d->d_mu, (int)d->B, B_pad, d->tau, d->sigma, pl->N, M, pl->Mh,
                           pl->ref_windows, pl->fa.ct, pl->stream));
     KCHK(soik_make_twiddles(pl->twS, S, S, 1, pl->stream));
@@ -521,6 +549,19 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       KCHK(soik_make_twiddles(pl->tw2, pl->n2, pl->n2, 1, pl->stream));
       KCHK(soik_make_twiddles(pl->thi, nhi, pl->Mh, pl->B1, pl->stream));
       KCHK(soik_make_twiddles(pl->tlo, pl->B1, pl->Mh, 1, pl->stream));
+    }
+    if (!pl->f32) {
+      /* The complex128 filter contraction on MFMA (v_mfma_f64_16x16x4_f64) where the shape
+       * allows: the default.  SOI_FILTER_MFMA64=0 selects the VALU form. */
+      const char *em = getenv("SOI_FILTER_MFMA64");
+      if ((!em || atoi(em) != 0) &&
+          soik_filter_mfma_supported((int)S, d->n_mu, d->d_mu, pl->fa.ct, B_pad, 3)) {
+        HIPCHK(hipMalloc(&pl->wt, soik_filter_mfma_table_bytes((int)S, B_pad, 3)));
+        KCHK(soik_filter_mfma_table((const double *)pl->taps, pl->wt, (int)S, B_pad, 3,
+                                    pl->stream));
+        pl->fa.wt = pl->wt;
+        pl->fa.mfma = 3;
+      }
     }
     if (pl->f32) {
       /* FP32 copies of the tables, rounded from the FP64 values generated above */
@@ -565,7 +606,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     }
     if (d->use_vlc) {
       long nch = S * d->P;
-      HIPCHK(hipMalloc((void **)&d->delta, (size_t)nch * 4 * pl->l * sizeof(int) + 256));
+      HIPCHK(hipMalloc((void **)&pl->delta, (size_t)nch * 4 * pl->l * sizeof(int) + 256));
       HIPCHK(hipMalloc((void **)&pl->d_exps, nch * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_emi, nch * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_nwords, nch * sizeof(int)));
@@ -575,18 +616,15 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     }
     HIPCHK(hipStreamSynchronize(pl->stream));
   }
+  d->alpha_tilde = (cfft_complex_t *)pl->at;
+  d->w = (cfft_complex_t *)pl->wtab;
+  d->W_inv = (cfft_complex_t *)pl->winv;
+  d->delta = pl->delta;
   d->plan = pl;
   return;
 fail:
 fail_quiet:
   if (pl) {
-    if (pl->owns_at && d->alpha_tilde) (void)hipFree(d->alpha_tilde), d->alpha_tilde = NULL;
-    (void)hipFree(d->w);
-    (void)hipFree(d->W_inv);
-    (void)hipFree(d->delta);
-    d->w = NULL;
-    d->W_inv = NULL;
-    d->delta = NULL;
     d->w_dup = NULL;
     free(d->segmentBoundaries);
     d->segmentBoundaries = NULL;
@@ -604,10 +642,6 @@ void free_soi_descriptor(soi_desc_t *d)
   }
   (void)hipSetDevice(pl->device);
   (void)hipStreamSynchronize(pl->stream);
-  (void)hipFree(d->w);
-  (void)hipFree(d->W_inv);
-  (void)hipFree(d->delta);
-  if (pl->owns_at) (void)hipFree(d->alpha_tilde);
   d->alpha_tilde = NULL;
   d->beta_tilde = NULL;
   d->gamma_tilde = NULL;
@@ -678,7 +712,7 @@ static void vlc_plan(soi_desc_t *d, soi_plan_t *pl)
 static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, SoiBatch b, hipStream_t st)
 {
   pl->fa.x = x;
-  pl->fa.at = d->alpha_tilde;
+  pl->fa.at = pl->at;
   pl->fa.b = b;
   int e = soik_filter_sdft(&pl->fa, st);
   pl->fa.b = (SoiBatch){0};
@@ -694,7 +728,7 @@ static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
   const int S = (int)pl->S, P = pl->P;
   const long l = pl->l;
   const int nch = S * P;
-  double *at = (double *)d->alpha_tilde;
+  double *at = (double *)pl->at;
   /* chunk (s, p) = alpha_tilde[s][p*l .. (p+1)*l): 2l doubles at at + 2*(s*Mh + p*l);
    * with Mh = P*l the chunks are contiguous in (s, p) order, stride 2l doubles */
   KCHK(soik_max_exponent(at, (int)(2 * l), nch, pl->d_exps, st));
@@ -713,9 +747,9 @@ static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
   for (int s = 0; s < S; ++s)
     for (int p = 0; p < P; ++p) pl->h_exps[s * P + p] = pl->seg_max[s];
   HIPCHK(hipMemcpyAsync(pl->d_emi, pl->h_exps, nch * sizeof(int), hipMemcpyHostToDevice, st));
-  KCHK(soik_compress(d->delta, 4 * l, at, (int)(2 * l), nch, pl->total_max, pl->d_emi,
+  KCHK(soik_compress(pl->delta, 4 * l, at, (int)(2 * l), nch, pl->total_max, pl->d_emi,
                      pl->d_nwords, st));
-  KCHK(soik_decompress(at, d->delta, 4 * l, (int)(2 * l), nch, pl->total_max, pl->d_emi, st));
+  KCHK(soik_decompress(at, pl->delta, 4 * l, (int)(2 * l), nch, pl->total_max, pl->d_emi, st));
   pl->vlc_valid = 1;
   return 0;
 fail:
@@ -724,8 +758,8 @@ fail:
 
 static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, SoiBatch b, hipStream_t st)
 {
-  soi_c32_t *at = (soi_c32_t *)d->alpha_tilde;
-  const soi_c64_t *winv = (const soi_c64_t *)d->W_inv;
+  soi_c32_t *at = (soi_c32_t *)pl->at;
+  const soi_c64_t *winv = pl->winv;
   if (pl->fft_single) {
     mark(pl, 2, st);
     KCHK(soik_fft_single_f32(at, X, pl->tw1_32, winv, pl->Mh, pl->M, (int)pl->S,
@@ -758,10 +792,10 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, SoiBatch b, hipStrea
 {
   if (pl->f32) return run_fft_f32(d, pl, (soi_c32_t *)Xv, b, st);
   soi_c64_t *X = (soi_c64_t *)Xv;
-  soi_c64_t *at = (soi_c64_t *)d->alpha_tilde;
+  soi_c64_t *at = (soi_c64_t *)pl->at;
   if (pl->fft_single) {
     mark(pl, 2, st);
-    KCHK(soik_fft_single(at, X, pl->tw1, (const soi_c64_t *)d->W_inv, pl->Mh, pl->M, (int)pl->S,
+    KCHK(soik_fft_single(at, X, pl->tw1, pl->winv, pl->Mh, pl->M, (int)pl->S,
                          pl->single_count, b, st));
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
@@ -770,16 +804,16 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, SoiBatch b, hipStrea
                            (int)pl->S, pl->Tc, pl->clay, gt, b, st));
     mark(pl, 3, st);
     if (gt)
-      KCHK(soik_fft_rows_tiled(gt, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
+      KCHK(soik_fft_rows_tiled(gt, X, pl->tw2, pl->winv, pl->n1, pl->n2,
                                pl->Mh, pl->M, (int)pl->S, b, st));
     else
-      KCHK(soik_fft_rows_reg(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2,
+      KCHK(soik_fft_rows_reg(at, X, pl->tw2, pl->winv, pl->n1, pl->n2,
                              pl->Mh, pl->M, (int)pl->S, pl->Tr, b, st));
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
                       (int)pl->S, b, st));
     mark(pl, 3, st);
-    KCHK(soik_fft_row(at, X, pl->tw2, (const soi_c64_t *)d->W_inv, pl->n1, pl->n2, pl->Tr,
+    KCHK(soik_fft_row(at, X, pl->tw2, pl->winv, pl->n1, pl->n2, pl->Tr,
                       pl->Mh, pl->M, (int)pl->S, b, st));
   }
   return 0;
@@ -1174,7 +1208,7 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
     for (int i = 0, m = 0; i < count; ++i) {
       if (grp[i] != g) continue;
       htab[m] = bufs[i];
-      htab[n + m] = descs[i]->alpha_tilde;
+      htab[n + m] = ((soi_plan_t *)descs[i]->plan)->at;
       htab[2 * n + m] = ((soi_plan_t *)descs[i]->plan)->tbuf;
       ++m;
     }

@@ -794,6 +794,253 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Complex128 real-tap filter on MFMA (v_mfma_f64_16x16x4_f64, FP64 products and accumulation).
+// The GEMM of each lane i is the one of the FP32 MFMA form with the rows ordered part-major,
+//   C[8 part + m][theta + 5D] = sum_k x_i[4 J0 + 12 m + k].part * t_i[theta][k - 4D],
+// m < 8, D < 3 (24 row groups J = J0 + 3m + D per tile), K = 80 >= BPAD + 8 in 20 four-deep
+// steps.  Lane layout (checked by soik_mfma64_layout_probe): A[row][k] in lane row + 16k,
+// B[k][col] in lane col + 16k, D[row][col] in lane col + 16 (row % 4), register row / 4.  With
+// part-major rows, lane col + 16g holds m = g (registers 0 and 2: real and imaginary part) and
+// m = 4 + g (registers 1 and 3).
+//
+// A complex128 tile (164 input rows, 82 KB at S = 32) and its output rows (61 KB) do not fit
+// twice in LDS, so the input rows live in a ring of RC one-KB chunks (the direct-to-LDS load of
+// one wave, each followed by a 16-byte gap): the tile of row groups [24t, 24t + 24) reads chunks
+// [AC t, AC t + RC) of the input, slot = chunk mod RC, and the AC chunks that tile t + 1 adds
+// overwrite the AC slots that only tile t reads.  Those loads are issued as soon as tile t's
+// MFMA block is done and complete under its post phase, S-point DFT and stores.  The B operands
+// (taps, 20 per lane i) stay in registers.
+// ---------------------------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int S, int LPW> struct Mfma64Cfg {
+  static constexpr int TJ = 24;                 // row groups per tile
+  static constexpr int NK = 20;                 // four-deep K steps
+  static constexpr int K = 4 * NK;
+  static constexpr int ROWS = 12 * 7 + K;       // input rows read per tile
+  static constexpr int ADV = TJ * 4;            // rows from one tile to the next
+  static constexpr int CH = 64;                 // complex128 values per wave-wide load (1 KB)
+  static constexpr int CP = CH + 1;             // chunk pitch in LDS: a 16-byte gap
+  static constexpr int RC = (ROWS * S + CH - 1) / CH;  // ring chunks: one tile's rows
+  static constexpr int AC = ADV * S / CH;       // chunks added per tile
+  static constexpr int NWAVES = S / LPW;
+  static constexpr int T = NWAVES * 64;
+  static constexpr int GR = TJ * 5;             // output rows R per tile
+  static constexpr int GS = S + 1;
+  static constexpr int NSC = (GR * S + T - 1) / T;
+  static constexpr int NSC_MIN = (GR * S) / T;
+  static_assert(ADV * S % CH == 0 && AC <= RC, "ring shape");
+  static constexpr int LDS_BYTES = (RC * CP + GR * GS + S + 5 * S) * 16;
+};
+
+constexpr int mfma64_threads(int S, int LPW) { return S / LPW * 64; }
+
+template <int S, int LPW>
+__global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 256) void k_filter_mfma64(
+    const double2 *__restrict__ x, double2 *__restrict__ at, const double *__restrict__ wt,
+    const double2 *__restrict__ post, const double2 *__restrict__ twS, long M, long Mh, long nJ,
+    int tiles_per_block, SoiBatch b)
+{
+  typedef Mfma64Cfg<S, LPW> CF;
+  x = batch_ptr(x, b.x);
+  at = batch_ptr(at, b.at);
+  constexpr int NK = CF::NK, CH = CF::CH, CP = CF::CP, RC = CF::RC, AC = CF::AC;
+  __shared__ __align__(16) double2 ring[RC * CP];
+  __shared__ __align__(16) double2 G[CF::GR * CF::GS];
+  __shared__ __align__(16) double2 tw[S];
+  __shared__ __align__(16) double2 pfl[5 * S];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int n = lane & 15, g = lane >> 4;
+
+  double wb[LPW][NK];  // B operands of this wave's lanes i
+#pragma unroll
+  for (int li = 0; li < LPW; ++li)
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((wave * LPW + li) * NK + ks) * 64 + lane];
+  for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
+  for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
+
+  const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
+  const long t0 = (long)blockIdx.x * tiles_per_block;
+  const long t1 = min(ntiles, t0 + tiles_per_block);
+  const long N = M * S;
+  if (t0 >= t1) return;
+
+  // input chunks [c0, c0 + cnt) into their ring slots (absolute chunk c -> slot c mod RC);
+  // a chunk is 64 consecutive values from element 64 c, wrapping at N
+  auto dma = [&](long c0, int cnt) {
+    int slot = (int)((c0 + wave) % RC);
+    for (int c = wave; c < cnt; c += CF::NWAVES) {
+      long gi = (c0 + c) * CH + lane;
+      if (gi >= N) gi -= N;
+      __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(ring + slot * CP), 16,
+                                       0, 0);
+      slot += CF::NWAVES;
+      if (slot >= RC) slot -= RC;
+    }
+  };
+
+  dma(t0 * AC, RC);
+  wait_vmcnt<0>();
+  lds_barrier();
+
+  // A operand of lane (n, g): row (m, part) = (n & 7, n >> 3), k = g; value index within the
+  // tile (12 m + 4 ks + g) * S + i
+  const int am = n & 7, apart = n >> 3;
+  const int v0 = (12 * am + g) * S + wave * LPW;
+  int base = (int)((t0 * AC) % RC);  // ring slot of the tile's first chunk
+  const double *ringd = reinterpret_cast<const double *>(ring);
+  for (long tile = t0; tile < t1; ++tile) {
+    const bool more = tile + 1 < t1;
+    // the epilogue's per-thread indices and addresses are recomputed each tile from an opaque
+    // copy of the thread index: hoisted out of the tile loop they would stay live through the
+    // MFMA block, beside the 20 B operands per lane i
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+    const int ln = tv & 63, nv = ln & 15, gv = ln >> 4;
+
+    // ---- MFMA: one accumulation chain per lane i, all on v_mfma_f64_16x16x4_f64
+    // A operand addresses.  For S >= 16 a K step advances by whole chunks (4S/64) and the lanes
+    // i of a wave stay inside one chunk, so a read is the lane's first address of the tile plus
+    // a compile-time offset, minus the ring length once the lane's chunk wraps.  (Per-read
+    // offsets computed once for the kernel would be loop-invariant and hoisted: 80 live
+    // registers.)  Smaller S take the chunk and offset of each read from its value index.
+    constexpr int CPB = CP * 16;            // chunk pitch in bytes
+    int a0 = 0, a0w = 0, wrap = 0;
+    if constexpr (S >= 16) {
+      int sb = base + (v0 >> 6);
+      if (sb >= RC) sb -= RC;
+      a0 = sb * CPB + (v0 & 63) * 16 + apart * 8;
+      a0w = a0 - RC * CPB;
+      wrap = RC - sb;
+    }
+    const char *ringb = reinterpret_cast<const char *>(ring);
+    auto aread = [&](int ks, int li) -> double {
+      if constexpr (S >= 16) {
+        constexpr int CS = 4 * S / 64;      // chunks per K step
+        const int cs = ks * CS;
+        const int ab = cs >= wrap ? a0w : a0;
+        return *reinterpret_cast<const double *>(ringb + ab + cs * CPB + li * 16);
+      } else {
+        const int v = v0 + 4 * ks * S + li;
+        int slot = base + (v >> 6);
+        if (slot >= RC) slot -= RC;
+        return ringd[2 * (slot * CP + (v & 63)) + apart];
+      }
+    };
+    // software pipeline: the A operands of step ks + 1 are read while step ks multiplies; the
+    // scheduling barriers keep every read from being hoisted to the top of the block
+    f64x4 acc[LPW];
+    double acur[LPW];
+#pragma unroll
+    for (int li = 0; li < LPW; ++li) {
+      acc[li] = f64x4{0, 0, 0, 0};
+      acur[li] = aread(0, li);
+    }
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+      double anxt[LPW];
+      if (ks + 1 < NK) {
+#pragma unroll
+        for (int li = 0; li < LPW; ++li) anxt[li] = aread(ks + 1, li);
+      }
+#pragma unroll
+      for (int li = 0; li < LPW; ++li)
+        acc[li] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[li], wb[li][ks], acc[li], 0, 0, 0);
+      if (ks + 1 < NK) {
+#pragma unroll
+        for (int li = 0; li < LPW; ++li) acur[li] = anxt[li];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    lds_barrier();  // every wave is done reading this tile's rows and the previous tile's G
+    if (more) dma((tile + 1) * AC + RC - AC, AC);  // the next tile's new chunks, into dead slots
+    base += AC;
+    if (base >= RC) base -= RC;
+
+    // ---- post phase into G: registers h and h + 2 of this lane are the real and imaginary
+    // part of row m = 4h + g
+    if (nv < 15) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        const int i = wave * LPW + li;
+        const double2 pf = pfl[(nv % 5) * S + i];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          G[(15 * (4 * h + gv) + nv) * CF::GS + i] = cmul(double2{acc[li][h], acc[li][h + 2]}, pf);
+      }
+    }
+    lds_barrier();
+
+    // ---- S-point DFT of each row
+    sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, double2>(G, tw, tv);
+
+    // ---- store segment-major: R = 5 J0 + r (runs of GR consecutive R per segment)
+    const long R0 = tile * CF::TJ * 5;
+#pragma unroll
+    for (int c = 0; c < CF::NSC; ++c) {
+      const int e = tv + c * CF::T;
+      if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
+        const int s = e / CF::GR, r = e % CF::GR;
+        if (tile * CF::TJ + r / 5 < nJ) store_nt(at + (long)s * Mh + R0 + r, G[r * CF::GS + s]);
+      }
+    }
+    if (more) {
+      // the next tile's chunks are complete once at most this wave's stores are outstanding
+      // (issued after the loads); a partial last tile stores fewer
+      asm volatile("" ::: "memory");
+      if ((tile + 1) * CF::TJ <= nJ) wait_vmcnt<CF::NSC_MIN>();
+      else wait_vmcnt<0>();
+    }
+    lds_barrier();
+  }
+}
+
+// FP64 B operand table: wt[(i*NK + ks)*64 + lane] = t_i[theta][k - 4D] for k = 4 ks + lane/16
+// and col = lane % 16 = theta + 5D (zero for col 15 or a tap index outside [0, BPAD)), from the
+// FP64 real-tap table taps[(theta*S + i)*BPAD + b].
+__global__ void k_filter_mfma64_table(const double *__restrict__ taps, double *__restrict__ wt, int S,
+                                      int BPAD, int NK)
+{
+  const long total = (long)S * NK * 64;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(e % 64), ks = (int)(e / 64 % NK), i = (int)(e / (64L * NK));
+    const int col = lane % 16, k = 4 * ks + lane / 16;
+    const int theta = col % 5, D = col / 5, bb = k - 4 * D;
+    const bool on = col < 15 && bb >= 0 && bb < BPAD;
+    wt[e] = on ? taps[((long)theta * S + i) * BPAD + bb] : 0.0;
+  }
+}
+
+template <int S, int LPW> static int launch_mfma64(const SoiFilterArgs *a, hipStream_t st)
+{
+  typedef Mfma64Cfg<S, LPW> CF;
+  static_assert(CF::LDS_BYTES <= 160 * 1024, "complex128 MFMA filter exceeds LDS");
+  if ((long)CF::RC * CF::CH > a->M * S) return (int)hipErrorInvalidValue;  // wraps at most once
+  const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
+  long nb = 256;
+  if (nb > ntiles) nb = ntiles;
+  const int tpb = (int)((ntiles + nb - 1) / nb);
+  nb = (ntiles + tpb - 1) / tpb;
+  hipLaunchKernelGGL((k_filter_mfma64<S, LPW>), dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1),
+                     dim3(CF::T), 0, st, (const double2 *)a->x, (double2 *)a->at,
+                     (const double *)a->wt, (const double2 *)a->post, (const double2 *)a->twS, a->M,
+                     a->Mh, a->nJ, tpb, a->b);
+  return (int)hipGetLastError();
+}
+
+static int filter_mfma64(const SoiFilterArgs *a, hipStream_t st)
+{
+  switch (a->S) {
+    case 4: return launch_mfma64<4, 2>(a, st);
+    case 8: return launch_mfma64<8, 2>(a, st);
+    case 16: return launch_mfma64<16, 2>(a, st);
+    case 32: return launch_mfma64<32, 4>(a, st);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
 // FP32 B operand table: wt[(i*NK + ks)*64 + lane] = t_i[theta][k - 4D] for k = 4 ks + lane/16
 // and n = lane % 16 = theta + 5D (zero for n = 15 or a tap index outside [0, BPAD)), rounded
 // once from the FP64 real-tap table taps[(theta*S + i)*BPAD + b].
@@ -841,14 +1088,15 @@ static int filter_mfma32(const SoiFilterArgs *a, hipStream_t st)
 // most 88 (split-bf16, mode 1) or 72 (FP32, mode 2) padded taps.
 extern "C" int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD, int mode)
 {
-  const int maxb = mode == 2 ? Mfma32Cfg<4>::K - 8 : 88;
-  return (mode == 1 || mode == 2) && !ct && n_mu == 5 && d_mu == 4 && BPAD <= maxb &&
+  const int maxb = mode == 1 ? 88 : Mfma32Cfg<4>::K - 8;
+  return (mode == 1 || mode == 2 || mode == 3) && !ct && n_mu == 5 && d_mu == 4 && BPAD <= maxb &&
          (S == 4 || S == 8 || S == 16 || S == 32);
 }
 
 extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD, int mode)
 {
   (void)BPAD;
+  if (mode == 3) return (size_t)S * Mfma64Cfg<4, 2>::NK * 64 * sizeof(double);
   if (mode == 2) return (size_t)S * Mfma32Cfg<4>::NK * 64 * sizeof(float);
   return (size_t)S * MfmaCfg<4>::NCH * 3 * 64 * 16;
 }
@@ -856,7 +1104,10 @@ extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD, int mode)
 extern "C" int soik_filter_mfma_table(const double *taps, void *wt, int S, int BPAD, int mode,
                                       hipStream_t st)
 {
-  if (mode == 2)
+  if (mode == 3)
+    hipLaunchKernelGGL(k_filter_mfma64_table, dim3(64), dim3(256), 0, st, taps, (double *)wt, S,
+                       BPAD, Mfma64Cfg<4, 2>::NK);
+  else if (mode == 2)
     hipLaunchKernelGGL(k_filter_mfma32_table, dim3(64), dim3(256), 0, st, taps, (float *)wt, S,
                        BPAD, Mfma32Cfg<4>::NK);
   else
@@ -874,6 +1125,39 @@ static int filter_mfma(const SoiFilterArgs *a, hipStream_t st)
     case 32: return launch_mfma<32>(a, st);
     default: return (int)hipErrorInvalidValue;
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lane layout check of v_mfma_f64_16x16x4_f64, the instruction of the complex128 MFMA filter.
+// One wave.  Experiments e < 64: A is one-hot at lane e (1.0), B holds lane + 1 in lane, C = 0.
+// Experiment 64: small integers in A, B and C (every product and sum exact in FP64).  Output
+// out[e*256 + lane*4 + r] = result register r of lane; 65*256 doubles.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mfma64_layout_probe(double *__restrict__ out)
+{
+  const int lane = threadIdx.x;
+  for (int e = 0; e < 64; ++e) {
+    const double a = lane == e ? 1.0 : 0.0;
+    const double b = (double)(lane + 1);
+    f64x4 acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[e * 256 + lane * 4 + r] = acc[r];
+  }
+  const double a = (double)((lane * 7) % 11 - 5);
+  const double b = (double)((lane * 5) % 13 - 6);
+  f64x4 acc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = (double)(100 * lane + r);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[64 * 256 + lane * 4 + r] = acc[r];
+}
+
+extern "C" int soik_mfma64_layout_probe(double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_mfma64_layout_probe, dim3(1), dim3(64), 0, st, out);
+  return (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -986,6 +1270,7 @@ extern "C" size_t soik_filter_lds_bytes(const SoiFilterArgs *a)
 
 extern "C" int soik_filter_sdft(const SoiFilterArgs *a, hipStream_t st)
 {
+  if (a->wt && a->mfma == 3 && !a->f32) return filter_mfma64(a, st);
   if (a->f32 && a->wt) return a->mfma == 2 ? filter_mfma32(a, st) : filter_mfma(a, st);
   if (!a->ct) {
     RT_CASE(4, 32, 16) RT_CASE(4, 36, 16) RT_CASE(4, 40, 16)

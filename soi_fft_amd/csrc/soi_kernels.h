@@ -35,10 +35,10 @@ typedef struct SoiFilterArgs {
   const void *taps;   /* [n_mu][S][NP*BP], real (ct = 0) or complex (ct = 1), double or float */
   const void *post;   /* [n_mu][S] post phase, complex of the same precision */
   const void *twS;    /* S-point twiddles, complex of the same precision */
-  const void *wt;     /* complex64 only: tap operands of an MFMA form (NULL: VALU form) */
+  const void *wt;     /* tap operands of an MFMA form (NULL: VALU form) */
   long M, Mh, nJ;
   int S, n_mu, d_mu, BP, NP, NT, U, W, ct, f32;
-  int mfma;           /* MFMA form of wt: 1 = split-bf16 pieces, 2 = FP32 */
+  int mfma;           /* MFMA form of wt: 1 = split-bf16 pieces, 2 = FP32, 3 = FP64 (complex128) */
   SoiBatch b;         /* batched launch tables (x -> b.x[t], at -> b.at[t]) */
 } SoiFilterArgs;
 

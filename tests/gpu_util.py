@@ -31,3 +31,21 @@ def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def hip_malloc(nbytes):
+    """A plain hipMalloc allocation (its own address range, unlike torch's caching allocator)."""
+    h = hip()
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMalloc.restype = ctypes.c_int
+    p = ctypes.c_void_p()
+    rc = h.hipMalloc(ctypes.byref(p), nbytes)
+    assert rc == 0, rc
+    return p.value
+
+
+def hip_free(ptr):
+    h = hip()
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    h.hipFree.restype = ctypes.c_int
+    assert h.hipFree(ptr) == 0

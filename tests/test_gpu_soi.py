@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from closed_forms import tones_dft
-from gpu_util import d2h, rel, to_dev
+from gpu_util import d2h, hip_free, hip_malloc, rel, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,62 @@ def test_filter_stage_matches_oracle(soi, orc, torch_cuda):
         assert np.array_equal(xt.cpu().numpy(), x)
     finally:
         soi.free_soi_descriptor(d)
+
+
+def test_init_checks_preallocated_alpha_tilde(soi, orc, torch_cuda):
+    """alpha_tilde holds all S segments here ([S][M_hat]); a buffer sized as the reference's
+    per-rank M_hat*k (pfft.c:161-163) is rejected at init, before any kernel runs."""
+    N, P = 2 ** 16, 8
+    d = soi.SoiDesc()
+    soi.set_default_soi_descriptor(d)
+    d.N, d.P = N, P
+    small = hip_malloc(d.M_hat * 16)
+    try:
+        d.alpha_tilde = small
+        with pytest.raises(soi.SoiError) as e:
+            soi.init_soi_descriptor(d, 1)
+        assert e.value.status == soi.SOI_ERR_ARGS
+        assert "alpha_tilde" in str(e.value)
+        assert not d.plan
+    finally:
+        hip_free(small)
+    full = hip_malloc(d.S * d.M_hat * 16)
+    try:
+        d.alpha_tilde = full
+        soi.init_soi_descriptor(d, 1)
+        x = orc.uniform_input(N, 17)
+        xt = to_dev(torch_cuda, x)
+        soi.compute_soi(d, xt.data_ptr())
+        ref, _ = orc.compute_soi(orc.params(N, P), x)
+        assert rel(xt.cpu().numpy(), ref) < TOL
+        soi.free_soi_descriptor(d)
+    finally:
+        hip_free(full)
+
+
+def test_reference_caller_nulls_buffers(soi, orc, torch_cuda):
+    """test.c:406-420 nulls alpha_tilde, beta_tilde and gamma_tilde (and frees w, W_inv) before
+    free_soi_descriptor: the plan keeps what it owns, so a second transform through the same
+    descriptor still works and free releases the buffers."""
+    N, P = 2 ** 20, 8
+    x = orc.uniform_input(N, 23)
+    ref, _ = orc.compute_soi(orc.params(N, P), x)
+    xt = to_dev(torch_cuda, x)
+    d = soi.make_descriptor(N, P)
+    torch_cuda.cuda.synchronize()
+    free0 = torch_cuda.cuda.mem_get_info()[0]
+    for _ in range(2):
+        xt.copy_(torch_cuda.from_numpy(x))
+        soi.compute_soi(d, xt.data_ptr())
+        assert rel(xt.cpu().numpy(), ref) < TOL
+        d.alpha_tilde = None
+        d.beta_tilde = None
+        d.gamma_tilde = None
+        d.w = None
+        d.W_inv = None
+    soi.free_soi_descriptor(d)
+    # alpha_tilde (S*M_hat complex128 = 20 MiB here) and W_inv (2 MiB) were released
+    assert torch_cuda.cuda.mem_get_info()[0] >= free0 + d.S * d.M_hat * 16
 
 
 def test_window_tables_match_oracle(soi, orc, torch_cuda):

@@ -34,6 +34,18 @@ int soi_compress_batch(int *out, long stride_words, const double *x, int len, in
 int soi_decompress_batch(double *out, const int *in, long stride_words, int len, int nchunk,
                          int e_max, const int *e_max_i, void *stream);
 
+/* No counterpart in the reference.  An allocation-free compress(), for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the batched form of compress() over nchunk chunks of len doubles (chunk c
+ * at x + c*len, its words at out + c*stride_words), x and out device pointers, e_max_i and
+ * nwords host arrays of nchunk ints (the segment exponents in, compress()'s return value per
+ * chunk out).  The exponents and counts pass through a static device scratch of at most 4096
+ * chunks (more is rejected with status 1); nothing is allocated.  Synchronises the stream
+ * (hipStream_t, NULL = default) before returning.  Returns 0 or a soi.h status code
+ * (soi_last_error describes it).  Calls that overlap on one device share the scratch and race. */
+int soi_compress_noalloc(int *out, long stride_words, const double *x, int len, int nchunk,
+                         int e_max, const int *e_max_i, int *nwords, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

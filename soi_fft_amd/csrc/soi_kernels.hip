@@ -638,3 +638,72 @@ extern "C" int soik_f64_to_f32(const double2 *in, float2 *out, long n, hipStream
   hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n, 256)), dim3(256), 0, st, in, out, n);
   return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free compress (soi_compress_noalloc in compress.h): the per-chunk segment exponents
+// and word counts that compress() stages through hipMalloc'ed buffers live in a file-scope
+// device array instead.  Its device address comes from hipGetSymbolAddress, once per device on
+// the first call there (the array's name is a host-side stand-in in host code and is never
+// passed to a kernel or a copy).  Every call on a device shares that device's array: calls
+// that overlap on one device race.  The kernel writes every word count it reports, and the
+// exponents are copied in before the kernel runs, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------------------------
+#include <mutex>
+
+#define SOI_NOALLOC_MAX_CHUNKS 4096
+#define SOI_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) int scratch[2 * SOI_NOALLOC_MAX_CHUNKS];
+static int *g_scratch_ptr[SOI_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+extern "C" void soi_internal_set_error(int status, const char *msg);
+
+static int *scratch_for_current_device(void)
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SOI_NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<int *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int soi_compress_noalloc(int *out, long stride_words, const double *x, int len,
+                                    int nchunk, int e_max, const int *e_max_i, int *nwords,
+                                    void *stream)
+{
+  if (!out || !x || !e_max_i || !nwords || len < 0 || nchunk < 1 ||
+      stride_words < (long)len * 56 / 32 + 32) {
+    soi_internal_set_error(1, "soi_compress_noalloc: needs device out and x, host e_max_i and "
+                              "nwords, nchunk >= 1, stride_words >= len*56/32 + 32");
+    return 1;
+  }
+  if (nchunk > SOI_NOALLOC_MAX_CHUNKS) {
+    soi_internal_set_error(1, "soi_compress_noalloc: nchunk exceeds the scratch maximum of 4096 "
+                              "chunks");
+    return 1;
+  }
+  int *ptr = scratch_for_current_device();
+  if (!ptr) {
+    soi_internal_set_error(5, "soi_compress_noalloc: hipGetSymbolAddress of the scratch failed");
+    return 5;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int *emi = ptr, *nw = ptr + SOI_NOALLOC_MAX_CHUNKS;
+  int e = (int)hipMemcpyAsync(emi, e_max_i, sizeof(int) * nchunk, hipMemcpyHostToDevice, st);
+  if (!e) e = soik_compress(out, stride_words, x, len, nchunk, e_max, emi, nw, st);
+  if (!e) e = (int)hipMemcpyAsync(nwords, nw, sizeof(int) * nchunk, hipMemcpyDeviceToHost, st);
+  if (!e) e = (int)hipStreamSynchronize(st);
+  if (e) {
+    soi_internal_set_error(5, hipGetErrorString((hipError_t)e));
+    return 5;
+  }
+  soi_internal_set_error(0, "");
+  return 0;
+}

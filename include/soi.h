@@ -103,7 +103,9 @@ typedef struct {
   void *plan;                 /* opaque GPU plan (replaces desc_dft_s / desc_dft_m_hat) */
   void *sendRequests_, *recvRequests_;  /* layout slots for the reference's request arrays */
   int use_vlc;                /* variable length compression of the exchanged data */
-  int *segmentBoundaries;     /* host, P+1 entries: first segment of each rank */
+  int *segmentBoundaries;     /* host, P+1 entries: first segment of each rank (with use_vlc
+                                 the cost-balanced boundaries of the last run, pfft.c:406-429,
+                                 valid once that run's stream has completed) */
   double comm_to_comp_cost_ratio;
   int precision;              /* SOI_PREC_FP64 (default) or SOI_PREC_FP32 */
   int window_mode;            /* SOI_WINDOW_AUTO (default), _REFERENCE or _EXACT */
@@ -140,7 +142,8 @@ int soi_last_status(void);
 
 /* compute_soi on a caller-chosen HIP stream (hipStream_t passed as void*; NULL = the
  * descriptor's own stream) without a final synchronisation: returns after enqueueing when
- * alpha_dt is a device pointer. */
+ * alpha_dt is a device pointer, with use_vlc too (the VLC plan is computed on the device and
+ * copied to segmentBoundaries asynchronously). */
 void compute_soi_async(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream);
 
 /* Stream the descriptor's kernels run on (hipStream_t as void*). */
@@ -153,6 +156,14 @@ void *soi_stream(soi_desc_t *d);
  * parallel_filter_subsampling).  Returns the number of stages written, 0 if none was recorded. */
 void soi_enable_timing(soi_desc_t *d, int on);
 int soi_stage_times(soi_desc_t *d, double *ms, int max_stages);
+
+/* Form of the plan's filter contraction (not in the reference): SOI_FILTER_VALU (FP64 or FP32
+ * FMAs on the vector units), SOI_FILTER_MFMA_BF16X3 (complex64, split-bf16 MFMA),
+ * SOI_FILTER_MFMA_F32 (complex64, v_mfma_f32_16x16x4_f32) or SOI_FILTER_MFMA_F64 (complex128,
+ * v_mfma_f64_16x16x4_f64); -1 for a descriptor that is not initialised. */
+enum { SOI_FILTER_VALU = 0, SOI_FILTER_MFMA_BF16X3 = 1, SOI_FILTER_MFMA_F32 = 2,
+       SOI_FILTER_MFMA_F64 = 3 };
+int soi_filter_form(soi_desc_t *d);
 
 /* Per-segment maximum exponents and their global maximum from the last VLC run
  * (pfft.c:319-343), and the per-segment send counts in doubles (pfft.c:404). */

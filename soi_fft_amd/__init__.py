@@ -112,6 +112,8 @@ def lib():
         L.soi_enable_timing.argtypes = [P, ctypes.c_int]
         L.soi_stage_times.argtypes = [P, dp, ctypes.c_int]
         L.soi_stage_times.restype = ctypes.c_int
+        L.soi_filter_form.argtypes = [P]
+        L.soi_filter_form.restype = ctypes.c_int
         L.soi_vlc_exponents.argtypes = [P, ip, ip, ip]
         L.soi_vlc_exponents.restype = ctypes.c_int
         L.max_exponent.argtypes = [vp, ctypes.c_int]
@@ -200,6 +202,16 @@ def stage_times(d):
     n = lib().soi_stage_times(ctypes.byref(d), buf, 4)
     _check()
     return list(buf[:n])
+
+
+# filter forms (soi_filter_form)
+SOI_FILTER_VALU, SOI_FILTER_MFMA_BF16X3, SOI_FILTER_MFMA_F32, SOI_FILTER_MFMA_F64 = 0, 1, 2, 3
+
+
+def filter_form(d):
+    f = lib().soi_filter_form(ctypes.byref(d))
+    _check()
+    return f
 
 
 def vlc_exponents(d):

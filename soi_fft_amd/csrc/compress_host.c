@@ -31,14 +31,15 @@ int soi_max_exponent_batch(const double *x, int len, int nchunk, int *exps, void
 int soi_compress_batch(int *out, long stride_words, const double *x, int len, int nchunk,
                        int e_max, const int *e_max_i, int *nwords, void *stream)
 {
-  return soik_compress(out, stride_words, x, len, nchunk, e_max, e_max_i, nwords,
+  return soik_compress(out, stride_words, x, len, nchunk, e_max, e_max_i, nwords, NULL,
                        (hipStream_t)stream);
 }
 
 int soi_decompress_batch(double *out, const int *in, long stride_words, int len, int nchunk,
                          int e_max, const int *e_max_i, void *stream)
 {
-  return soik_decompress(out, in, stride_words, len, nchunk, e_max, e_max_i, (hipStream_t)stream);
+  return soik_decompress(out, in, stride_words, len, nchunk, e_max, e_max_i, NULL,
+                         (hipStream_t)stream);
 }
 
 /* words compress() writes for len doubles: vector rows then the scalar tail (compress.c:205-372) */
@@ -87,7 +88,7 @@ int compress(int *out, const double *in, int len, int e_max, int e_max_i, int pr
     if (hipMalloc((void **)&dout, (size_t)max_words(len) * sizeof(int)) != hipSuccess) goto done;
     dst = dout;
   }
-  if (soik_compress(dst, max_words(len), src, len, 1, e_max, demi, dnw, 0) == 0) {
+  if (soik_compress(dst, max_words(len), src, len, 1, e_max, demi, dnw, NULL, 0) == 0) {
     (void)hipMemcpy(&nw, dnw, sizeof(int), hipMemcpyDeviceToHost);
     if (!dev_out && nw > 0) (void)hipMemcpy(out, dout, (size_t)nw * sizeof(int), hipMemcpyDeviceToHost);
   }
@@ -130,7 +131,7 @@ void decompress(double *out, const int *in, int len, int e_max, int e_max_i, con
     if (hipMalloc((void **)&dout, (size_t)len * sizeof(double) + 8) != hipSuccess) goto done;
     dst = dout;
   }
-  if (soik_decompress(dst, src, max_words(len), len, 1, e_max, demi, 0) == 0 && !dev_out)
+  if (soik_decompress(dst, src, max_words(len), len, 1, e_max, demi, NULL, 0) == 0 && !dev_out)
     (void)hipMemcpy(out, dout, (size_t)len * sizeof(double), hipMemcpyDeviceToHost);
   else
     (void)hipDeviceSynchronize();

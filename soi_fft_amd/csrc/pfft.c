@@ -112,10 +112,14 @@ typedef struct {
   int owns_at;
   soi_c64_t *wtab, *winv;
   int *delta;
-  /* vlc */
-  int *d_exps, *d_emi, *d_nwords;
-  int *h_exps, *seg_max, *send_counts, total_max;
+  /* vlc: per-chunk exponents, segment exponents and word counts on the device; the plan of
+   * k_vlc_plan ([total, seg_max[S], send_counts[S], boundaries[P+1]]) on the device and, copied
+   * after each run, in pinned host memory (the descriptor's segmentBoundaries points into it) */
+  int *d_exps, *d_emi, *d_nwords, *d_vlc;
+  double *d_pcs;
+  int *h_vlc;
   int vlc_valid;
+  hipStream_t last_stream; /* stream of the last compute call */
   /* timing: NSTAGE+1 events per call, in a ring of NTSLOT calls */
   int timing;
   long ncalls;          /* calls recorded since timing was enabled */
@@ -391,9 +395,9 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->d_exps);
   (void)hipFree(pl->d_emi);
   (void)hipFree(pl->d_nwords);
-  free(pl->h_exps);
-  free(pl->seg_max);
-  free(pl->send_counts);
+  (void)hipFree(pl->d_vlc);
+  (void)hipFree(pl->d_pcs);
+  if (pl->h_vlc) (void)hipHostFree(pl->h_vlc);
   for (int c = 0; c < NTSLOT; ++c)
     for (int e = 0; e <= NSTAGE; ++e)
       if (pl->ev[c][e]) (void)hipEventDestroy(pl->ev[c][e]);
@@ -606,13 +610,20 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
     }
     if (d->use_vlc) {
       long nch = S * d->P;
+      size_t nvlc = (size_t)(1 + 2 * S + d->P + 1);
       HIPCHK(hipMalloc((void **)&pl->delta, (size_t)nch * 4 * pl->l * sizeof(int) + 256));
       HIPCHK(hipMalloc((void **)&pl->d_exps, nch * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_emi, nch * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_nwords, nch * sizeof(int)));
-      pl->h_exps = (int *)malloc(nch * sizeof(int));
-      pl->seg_max = (int *)malloc(S * sizeof(int));
-      pl->send_counts = (int *)malloc(S * sizeof(int));
+      HIPCHK(hipMalloc((void **)&pl->d_vlc, nvlc * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&pl->d_pcs, (size_t)(S + 1) * sizeof(double)));
+      HIPCHK(hipHostMalloc((void **)&pl->h_vlc, nvlc * sizeof(int), hipHostMallocDefault));
+      /* segmentBoundaries lives in the pinned plan copy, so that the run's device plan reaches
+       * it by an asynchronous copy */
+      int *sb = pl->h_vlc + 1 + 2 * S;
+      memcpy(sb, d->segmentBoundaries, sizeof(int) * (d->P + 1));
+      free(d->segmentBoundaries);
+      d->segmentBoundaries = sb;
     }
     HIPCHK(hipStreamSynchronize(pl->stream));
   }
@@ -626,7 +637,7 @@ fail:
 fail_quiet:
   if (pl) {
     d->w_dup = NULL;
-    free(d->segmentBoundaries);
+    if (!pl->h_vlc || d->segmentBoundaries != pl->h_vlc + 1 + 2 * pl->S) free(d->segmentBoundaries);
     d->segmentBoundaries = NULL;
     free_plan(pl);
   }
@@ -650,7 +661,7 @@ void free_soi_descriptor(soi_desc_t *d)
   d->w_dup = NULL;
   d->delta = NULL;
   d->epsilon = NULL;
-  free(d->segmentBoundaries);
+  if (!pl->h_vlc) free(d->segmentBoundaries); /* else it points into the plan's pinned copy */
   d->segmentBoundaries = NULL;
   free_plan(pl);
   d->plan = NULL;
@@ -673,42 +684,6 @@ static void mark(soi_plan_t *pl, int e, hipStream_t st)
   if (pl->timing) (void)hipEventRecord(pl->ev[pl->ncalls % NTSLOT][e], st);
 }
 
-/* pfft.c:390-440 restated: per-segment send counts and cost-balanced segment boundaries */
-static void vlc_plan(soi_desc_t *d, soi_plan_t *pl)
-{
-  int S = (int)pl->S, P = pl->P, k = pl->k;
-  long l = pl->l, Mh = pl->Mh;
-  int totalMax = pl->total_max;
-  double *pcs = (double *)malloc(sizeof(double) * (S + 1));
-  pcs[0] = 0;
-  for (int p = 0; p < P; ++p)
-    for (int ik = 0; ik < k; ++ik) {
-      int s = p * k + ik;
-      int bitsPerInput = 52 + 3 - (totalMax - pl->seg_max[s]);
-      if (bitsPerInput > 31) ++bitsPerInput;
-      int lenPerSimdLane = (int)((l * 2 / 16 * bitsPerInput + 31) / 32);
-      int remainder = (int)((l * 2 % 16 * bitsPerInput + 31) / 32);
-      int sendLen = lenPerSimdLane * 16 + remainder;
-      if (bitsPerInput <= 0) sendLen = 0;
-      pl->send_counts[s] = (sendLen + 1) / 2;
-      pcs[s + 1] = pcs[s] + d->comm_to_comp_cost_ratio * pl->send_counts[s] / 2 / Mh + 1;
-    }
-  double costPerRank = pcs[S] / P;
-  int s = 1;
-  for (int p = 1; p < P;) {
-    if (pcs[s] >= p * costPerRank) {
-      if (pcs[s] - p * costPerRank > p * costPerRank - pcs[s - 1])
-        d->segmentBoundaries[p] = s - 1;
-      else
-        d->segmentBoundaries[p] = s;
-      ++p;
-    } else {
-      ++s;
-    }
-  }
-  free(pcs);
-}
-
 static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, SoiBatch b, hipStream_t st)
 {
   pl->fa.x = x;
@@ -723,6 +698,11 @@ static int run_filter(soi_desc_t *d, soi_plan_t *pl, const void *x, SoiBatch b, 
   return 0;
 }
 
+/* The VLC branch (pfft.c:316-353, 390-440, 476-543) without leaving the device: per-chunk
+ * maximum exponents, then the plan (segment and global maxima, send counts, segment
+ * boundaries) in k_vlc_plan, compress and decompress reading the global maximum from device
+ * memory, and an asynchronous copy of the plan into pinned host memory (segmentBoundaries,
+ * soi_vlc_exponents). */
 static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
 {
   const int S = (int)pl->S, P = pl->P;
@@ -732,24 +712,13 @@ static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
   /* chunk (s, p) = alpha_tilde[s][p*l .. (p+1)*l): 2l doubles at at + 2*(s*Mh + p*l);
    * with Mh = P*l the chunks are contiguous in (s, p) order, stride 2l doubles */
   KCHK(soik_max_exponent(at, (int)(2 * l), nch, pl->d_exps, st));
-  HIPCHK(hipMemcpyAsync(pl->h_exps, pl->d_exps, nch * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  /* Allreduce(MAX) over ranks (pfft.c:331-333), then the global maximum (:340-343) */
-  pl->total_max = INT_MIN;
-  for (int s = 0; s < S; ++s) {
-    int m = INT_MIN;
-    for (int p = 0; p < P; ++p)
-      if (pl->h_exps[s * P + p] > m) m = pl->h_exps[s * P + p];
-    pl->seg_max[s] = m;
-    if (m > pl->total_max) pl->total_max = m;
-  }
-  vlc_plan(d, pl);
-  for (int s = 0; s < S; ++s)
-    for (int p = 0; p < P; ++p) pl->h_exps[s * P + p] = pl->seg_max[s];
-  HIPCHK(hipMemcpyAsync(pl->d_emi, pl->h_exps, nch * sizeof(int), hipMemcpyHostToDevice, st));
-  KCHK(soik_compress(pl->delta, 4 * l, at, (int)(2 * l), nch, pl->total_max, pl->d_emi,
-                     pl->d_nwords, st));
-  KCHK(soik_decompress(at, pl->delta, 4 * l, (int)(2 * l), nch, pl->total_max, pl->d_emi, st));
+  KCHK(soik_vlc_plan(pl->d_exps, S, P, pl->k, l, pl->Mh, d->comm_to_comp_cost_ratio, pl->d_emi,
+                     pl->d_vlc, pl->d_pcs, st));
+  KCHK(soik_compress(pl->delta, 4 * l, at, (int)(2 * l), nch, 0, pl->d_emi, pl->d_nwords,
+                     pl->d_vlc, st));
+  KCHK(soik_decompress(at, pl->delta, 4 * l, (int)(2 * l), nch, 0, pl->d_emi, pl->d_vlc, st));
+  HIPCHK(hipMemcpyAsync(pl->h_vlc, pl->d_vlc, sizeof(int) * (size_t)(1 + 2 * S + P + 1),
+                        hipMemcpyDeviceToHost, st));
   pl->vlc_valid = 1;
   return 0;
 fail:
@@ -836,6 +805,7 @@ static void compute_impl(soi_desc_t *d, cfft_complex_t *alpha_dt, void *stream, 
   }
   hipStream_t st = stream ? (hipStream_t)stream : pl->stream;
   HIPCHK(hipSetDevice(pl->device));
+  pl->last_stream = st;
   {
     int dev = is_device_ptr(alpha_dt);
     soi_c64_t *buf = (soi_c64_t *)alpha_dt;
@@ -1008,13 +978,26 @@ int soi_autotune(soi_desc_t *d, cfft_complex_t *scratch, int reps)
   return best;
 }
 
+int soi_filter_form(soi_desc_t *d)
+{
+  clear_err();
+  soi_plan_t *pl = d ? (soi_plan_t *)d->plan : NULL;
+  if (!pl) {
+    set_err(SOI_ERR_STATE, "descriptor is not initialised");
+    return -1;
+  }
+  return pl->fa.wt ? pl->fa.mfma : SOI_FILTER_VALU;
+}
+
 int soi_vlc_exponents(soi_desc_t *d, int *seg_max_exp, int *total_max_exp, int *send_counts)
 {
   soi_plan_t *pl = (soi_plan_t *)d->plan;
   if (!pl || !pl->vlc_valid) return 0;
-  if (seg_max_exp) memcpy(seg_max_exp, pl->seg_max, sizeof(int) * pl->S);
-  if (total_max_exp) *total_max_exp = pl->total_max;
-  if (send_counts) memcpy(send_counts, pl->send_counts, sizeof(int) * pl->S);
+  (void)hipSetDevice(pl->device);
+  (void)hipStreamSynchronize(pl->last_stream); /* the plan's copy of the last run has landed */
+  if (seg_max_exp) memcpy(seg_max_exp, pl->h_vlc + 1, sizeof(int) * pl->S);
+  if (total_max_exp) *total_max_exp = pl->h_vlc[0];
+  if (send_counts) memcpy(send_counts, pl->h_vlc + 1 + pl->S, sizeof(int) * pl->S);
   return (int)pl->S;
 }
 

@@ -21,6 +21,7 @@
 // loaded into registers while this tile computes, and barriers wait on LDS traffic only.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "soi_common.hpp"
 #include "soi_kernels.h"
 
@@ -804,15 +805,31 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
 // part-major rows, lane col + 16g holds m = g (registers 0 and 2: real and imaginary part) and
 // m = 4 + g (registers 1 and 3).
 //
+// Wave w of the NW = S / LPW waves owns the lanes i = NW li + w, li < LPW.  The S-point DFT of
+// each output row R then splits as i = NW li + w, s = s1 + LPW s2:
+//   X[s] = sum_w w_NW^{w s2} [ w_S^{w s1} sum_li gamma[NW li + w] w_LPW^{li s1} ],
+// the LPW-point DFTs in the registers of the wave that holds the lanes, then one exchange
+// through LDS and an NW-point DFT per (R, s1), whose NW results go straight to HBM.
+//
 // A complex128 tile (164 input rows, 82 KB at S = 32) and its output rows (61 KB) do not fit
 // twice in LDS, so the input rows live in a ring of RC one-KB chunks (the direct-to-LDS load of
 // one wave, each followed by a 16-byte gap): the tile of row groups [24t, 24t + 24) reads chunks
 // [AC t, AC t + RC) of the input, slot = chunk mod RC, and the AC chunks that tile t + 1 adds
 // overwrite the AC slots that only tile t reads.  Those loads are issued as soon as tile t's
-// MFMA block is done and complete under its post phase, S-point DFT and stores.  The B operands
-// (taps, 20 per lane i) stay in registers.
+// MFMA block is done and complete under its epilogue.  The B operands (taps, 20 per lane i)
+// stay in registers.
 // ---------------------------------------------------------------------------------------------
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2g __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) f64x2g gf64x2;
+
+// non-temporal store of one complex128 value through a global-address-space pointer: a flat
+// store would also count against LDS traffic, and the compiler then waits for the in-flight
+// direct-to-LDS loads before it
+__device__ __forceinline__ void store_nt_global(gf64x2 *p, double2 v)
+{
+  __builtin_nontemporal_store(f64x2g{v.x, v.y}, p);
+}
 
 template <int S, int LPW> struct Mfma64Cfg {
   static constexpr int TJ = 24;                 // row groups per tile
@@ -827,11 +844,16 @@ template <int S, int LPW> struct Mfma64Cfg {
   static constexpr int NWAVES = S / LPW;
   static constexpr int T = NWAVES * 64;
   static constexpr int GR = TJ * 5;             // output rows R per tile
-  static constexpr int GS = S + 1;
-  static constexpr int NSC = (GR * S + T - 1) / T;
-  static constexpr int NSC_MIN = (GR * S) / T;
+  static constexpr int GP = S + 1;              // exchange row pitch: [R][s1 * NWAVES + w]
+  static constexpr int NTASK = GR * LPW;        // (R, s1) tasks of the cross-wave DFT
+  static constexpr int NC = (NTASK + T - 1) / T;
+  // DUP: the idle lanes of the last task slot repeat tasks of their own wave (every wave then
+  // issues the same NC * NWAVES stores per tile); otherwise whole waves would be idle
+  static constexpr int DUPD = NC * T - NTASK;
+  static constexpr bool DUP = DUPD < 64;
   static_assert(ADV * S % CH == 0 && AC <= RC, "ring shape");
-  static constexpr int LDS_BYTES = (RC * CP + GR * GS + S + 5 * S) * 16;
+  static_assert(LPW == 2 || LPW == 4 || LPW == 8, "in-register DFT size");
+  static constexpr int LDS_BYTES = (RC * CP + GR * GP + S + 5 * S) * 16;
 };
 
 constexpr int mfma64_threads(int S, int LPW) { return S / LPW * 64; }
@@ -846,18 +868,20 @@ __global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 25
   x = batch_ptr(x, b.x);
   at = batch_ptr(at, b.at);
   constexpr int NK = CF::NK, CH = CF::CH, CP = CF::CP, RC = CF::RC, AC = CF::AC;
+  constexpr int NW = CF::NWAVES, GP = CF::GP;
+  gf64x2 *const atg = (gf64x2 *)at;
   __shared__ __align__(16) double2 ring[RC * CP];
-  __shared__ __align__(16) double2 G[CF::GR * CF::GS];
+  __shared__ __align__(16) double2 G[CF::GR * GP];
   __shared__ __align__(16) double2 tw[S];
   __shared__ __align__(16) double2 pfl[5 * S];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int n = lane & 15, g = lane >> 4;
 
-  double wb[LPW][NK];  // B operands of this wave's lanes i
+  double wb[LPW][NK];  // B operands of this wave's lanes i = NW li + wave
 #pragma unroll
   for (int li = 0; li < LPW; ++li)
 #pragma unroll
-    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((wave * LPW + li) * NK + ks) * 64 + lane];
+    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((li * NW + wave) * NK + ks) * 64 + lane];
   for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
   for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
 
@@ -871,12 +895,12 @@ __global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 25
   // a chunk is 64 consecutive values from element 64 c, wrapping at N
   auto dma = [&](long c0, int cnt) {
     int slot = (int)((c0 + wave) % RC);
-    for (int c = wave; c < cnt; c += CF::NWAVES) {
+    for (int c = wave; c < cnt; c += NW) {
       long gi = (c0 + c) * CH + lane;
       if (gi >= N) gi -= N;
       __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(ring + slot * CP), 16,
                                        0, 0);
-      slot += CF::NWAVES;
+      slot += NW;
       if (slot >= RC) slot -= RC;
     }
   };
@@ -886,13 +910,21 @@ __global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 25
   lds_barrier();
 
   // A operand of lane (n, g): row (m, part) = (n & 7, n >> 3), k = g; value index within the
-  // tile (12 m + 4 ks + g) * S + i
+  // tile (12 m + 4 ks + g) * S + i with i = NW li + wave
   const int am = n & 7, apart = n >> 3;
-  const int v0 = (12 * am + g) * S + wave * LPW;
+  const int v0 = (12 * am + g) * S + wave;
   int base = (int)((t0 * AC) % RC);  // ring slot of the tile's first chunk
   const double *ringd = reinterpret_cast<const double *>(ring);
-  for (long tile = t0; tile < t1; ++tile) {
-    const bool more = tile + 1 < t1;
+
+  // One tile.  PART: the transform's last tile when it holds fewer than TJ row groups (its
+  // stores are predicated per row; it is never followed by another tile of this block).
+  // Whole tiles issue the same store instructions in every wave, unconditionally (in DUP
+  // mode idle lanes repeat a neighbour's task and store the same value to the same address),
+  // so that the compiler's wait tracking sees every one of them and the ring reads of the
+  // next tile wait only for the loads issued before them, not for the stores issued after.
+  auto body = [&](long tile, auto part_tag) {
+    constexpr bool PART = decltype(part_tag)::value;
+    const bool more = !PART && tile + 1 < t1;
     // the epilogue's per-thread indices and addresses are recomputed each tile from an opaque
     // copy of the thread index: hoisted out of the tile loop they would stay live through the
     // MFMA block, beside the 20 B operands per lane i
@@ -921,9 +953,9 @@ __global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 25
         constexpr int CS = 4 * S / 64;      // chunks per K step
         const int cs = ks * CS;
         const int ab = cs >= wrap ? a0w : a0;
-        return *reinterpret_cast<const double *>(ringb + ab + cs * CPB + li * 16);
+        return *reinterpret_cast<const double *>(ringb + ab + cs * CPB + li * NW * 16);
       } else {
-        const int v = v0 + 4 * ks * S + li;
+        const int v = v0 + 4 * ks * S + li * NW;
         int slot = base + (v >> 6);
         if (slot >= RC) slot -= RC;
         return ringd[2 * (slot * CP + (v & 63)) + apart];
@@ -959,42 +991,63 @@ __global__ __launch_bounds__(mfma64_threads(S, LPW), mfma64_threads(S, LPW) / 25
     base += AC;
     if (base >= RC) base -= RC;
 
-    // ---- post phase into G: registers h and h + 2 of this lane are the real and imaginary
-    // part of row m = 4h + g
-    if (nv < 15) {
+    // ---- post phase and the LPW-point DFTs over this wave's lanes i, twiddled by w_S^{w s1},
+    // into the exchange rows: G[R][s1 * NW + w].  Registers h and h + 2 are the real and
+    // imaginary part of row m = 4h + g; column nv = theta + 5D (15 is the unused column).
 #pragma unroll
-      for (int li = 0; li < LPW; ++li) {
-        const int i = wave * LPW + li;
-        const double2 pf = pfl[(nv % 5) * S + i];
+    for (int h = 0; h < 2; ++h) {
+      double2 v[LPW];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          G[(15 * (4 * h + gv) + nv) * CF::GS + i] = cmul(double2{acc[li][h], acc[li][h + 2]}, pf);
+      for (int li = 0; li < LPW; ++li)
+        v[li] = cmul(double2{acc[li][h], acc[li][h + 2]}, pfl[(nv % 5) * S + li * NW + wave]);
+      dft_r<LPW>(v);
+      if (nv < 15) {
+        double2 *gr = G + (15 * (4 * h + gv) + nv) * GP + wave;
+        gr[0] = v[0];
+#pragma unroll
+        for (int s1 = 1; s1 < LPW; ++s1) gr[s1 * NW] = cmul(v[s1], tw[(wave * s1) % S]);
       }
     }
     lds_barrier();
 
-    // ---- S-point DFT of each row
-    sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, double2>(G, tw, tv);
-
-    // ---- store segment-major: R = 5 J0 + r (runs of GR consecutive R per segment)
+    // ---- NW-point DFT over the waves for each (R, s1), stored segment-major: X[s1 + LPW s2]
+    // of row R = 5 J0 + r goes to alpha_tilde[s1 + LPW s2][R]; consecutive lanes take
+    // consecutive r, so each store instruction writes runs of consecutive R
     const long R0 = tile * CF::TJ * 5;
 #pragma unroll
-    for (int c = 0; c < CF::NSC; ++c) {
-      const int e = tv + c * CF::T;
-      if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
-        const int s = e / CF::GR, r = e % CF::GR;
-        if (tile * CF::TJ + r / 5 < nJ) store_nt(at + (long)s * Mh + R0 + r, G[r * CF::GS + s]);
+    for (int c = 0; c < CF::NC; ++c) {
+      int e = tv + c * CF::T;
+      if constexpr (CF::DUP) {
+        if (e >= CF::NTASK) e -= CF::DUPD;
+      }
+      if (CF::DUP || CF::NTASK % CF::T == 0 || e < CF::NTASK) {
+        const int r = e % CF::GR, s1 = e / CF::GR;
+        double2 u[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) u[w] = G[r * GP + s1 * NW + w];
+        dft_r<NW>(u);
+        if (!PART || tile * CF::TJ + r / 5 < nJ) {
+#pragma unroll
+          for (int s2 = 0; s2 < NW; ++s2)
+            store_nt_global(atg + (long)(s1 + LPW * s2) * Mh + R0 + r, u[s2]);
+        }
       }
     }
-    if (more) {
-      // the next tile's chunks are complete once at most this wave's stores are outstanding
-      // (issued after the loads); a partial last tile stores fewer
+    // the next tile's chunks are complete once at most this wave's stores of this tile are
+    // outstanding (they were issued after the loads).  Unconditional on every path that loops
+    // back, so the compiler's wait tracking knows the ring is complete at the next tile.
+    if constexpr (!PART) {
       asm volatile("" ::: "memory");
-      if ((tile + 1) * CF::TJ <= nJ) wait_vmcnt<CF::NSC_MIN>();
+      if constexpr (CF::DUP) wait_vmcnt<CF::NC * NW>();
       else wait_vmcnt<0>();
     }
     lds_barrier();
-  }
+  };
+
+  // whole tiles in the loop; the transform's partial last tile, if this block has it, after it
+  const long tfull = min(t1, nJ / CF::TJ);
+  for (long tile = t0; tile < tfull; ++tile) body(tile, std::false_type{});
+  if (tfull < t1) body(tfull, std::true_type{});
 }
 
 // FP64 B operand table: wt[(i*NK + ks)*64 + lane] = t_i[theta][k - 4D] for k = 4 ks + lane/16

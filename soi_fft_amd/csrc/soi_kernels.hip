@@ -328,11 +328,11 @@ __device__ __forceinline__ uint32_t vlc_word(long pos, int fbits, long nfields, 
 // scalar tail.  nwords[c] = compress()'s return value.
 __global__ void k_compress(int *__restrict__ out, long stride, const double *__restrict__ x,
                            int len, int nchunk, int e_max, const int *__restrict__ emi,
-                           int *__restrict__ nwords)
+                           int *__restrict__ nwords, const int *__restrict__ e_max_dev)
 {
   const int c = blockIdx.y;
   if (c >= nchunk) return;
-  VlcParams v = vlc_params(e_max, emi[c]);
+  VlcParams v = vlc_params(e_max_dev ? *e_max_dev : e_max, emi[c]);
   const double *in = x + (long)c * len;
   int *o = out + (long)c * stride;
   if (v.i2Len <= 0) {
@@ -397,11 +397,12 @@ __device__ __forceinline__ double vlc_decode(uint64_t fv, const VlcParams &v, bo
 }
 
 __global__ void k_decompress(double *__restrict__ out, const int *__restrict__ in, long stride,
-                             int len, int nchunk, int e_max, const int *__restrict__ emi)
+                             int len, int nchunk, int e_max, const int *__restrict__ emi,
+                             const int *__restrict__ e_max_dev)
 {
   const int c = blockIdx.y;
   if (c >= nchunk) return;
-  VlcParams v = vlc_params(e_max, emi[c]);
+  VlcParams v = vlc_params(e_max_dev ? *e_max_dev : e_max, emi[c]);
   double *o = out + (long)c * len;
   const int *w = in + (long)c * stride;
   const long nvf = len / 16;
@@ -597,25 +598,91 @@ extern "C" int soik_max_exponent(const double *x, int len, int nchunk, int *out,
 }
 
 extern "C" int soik_compress(int *out, long stride, const double *x, int len, int nchunk,
-                             int e_max, const int *emi, int *nwords, hipStream_t st)
+                             int e_max, const int *emi, int *nwords, const int *e_max_dev,
+                             hipStream_t st)
 {
   if (nchunk <= 0) return 0;
   long maxw = (long)len * 56 / 32 / 1 + 64;
   unsigned gx = nblk(maxw, 256);
   if (gx > 4096) gx = 4096;
   hipLaunchKernelGGL(k_compress, dim3(gx, nchunk), dim3(256), 0, st, out, stride, x, len, nchunk,
-                     e_max, emi, nwords);
+                     e_max, emi, nwords, e_max_dev);
   return (int)hipGetLastError();
 }
 
 extern "C" int soik_decompress(double *out, const int *in, long stride, int len, int nchunk,
-                               int e_max, const int *emi, hipStream_t st)
+                               int e_max, const int *emi, const int *e_max_dev, hipStream_t st)
 {
   if (nchunk <= 0) return 0;
   unsigned gx = nblk(len, 256);
   if (gx > 4096) gx = 4096;
   hipLaunchKernelGGL(k_decompress, dim3(gx, nchunk), dim3(256), 0, st, out, in, stride, len,
-                     nchunk, e_max, emi);
+                     nchunk, e_max, emi, e_max_dev);
+  return (int)hipGetLastError();
+}
+
+// VLC planning on the device (pfft.c:316-343, 390-440): from the per-chunk exponents of the
+// S x P chunks (chunk (s, p) = s*P + p), the per-segment maximum (the reference's
+// MPI_Allreduce(MAX) over ranks), the global maximum, every chunk's segment exponent, the
+// per-segment send counts in doubles and the cost-balanced segment boundaries.  One block; the
+// boundary walk is sequential in thread 0 with the reference's double arithmetic, unfused, so
+// the boundaries are bit-identical to the host restatement.
+// out: [0] global maximum, [1 .. S] segment maxima, [1+S .. 2S] send counts,
+//      [1+2S .. 1+2S+P] segment boundaries; pcs: S + 1 doubles of scratch.
+__global__ void k_vlc_plan(const int *__restrict__ exps, int S, int P, int k, long l, long Mh,
+                           double ratio, int *__restrict__ emi, int *__restrict__ out,
+                           double *__restrict__ pcs)
+{
+#pragma clang fp contract(off)
+  __shared__ int smax;
+  if (threadIdx.x == 0) smax = INT_MIN;
+  __syncthreads();
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    int m = INT_MIN;
+    for (int p = 0; p < P; ++p) m = max(m, exps[s * P + p]);
+    out[1 + s] = m;
+    atomicMax(&smax, m);
+  }
+  __syncthreads();
+  const int total = smax;
+  for (int c = threadIdx.x; c < S * P; c += blockDim.x) emi[c] = out[1 + c / P];
+  if (threadIdx.x != 0) return;
+  out[0] = total;
+  pcs[0] = 0;
+  for (int p = 0; p < P; ++p)
+    for (int ik = 0; ik < k; ++ik) {
+      const int s = p * k + ik;
+      int bits = 52 + 3 - (total - out[1 + s]);
+      if (bits > 31) ++bits;
+      const int lenPerSimdLane = (int)((l * 2 / 16 * bits + 31) / 32);
+      const int remainder = (int)((l * 2 % 16 * bits + 31) / 32);
+      int sendLen = lenPerSimdLane * 16 + remainder;
+      if (bits <= 0) sendLen = 0;
+      const int cnt = (sendLen + 1) / 2;
+      out[1 + S + s] = cnt;
+      pcs[s + 1] = pcs[s] + ratio * cnt / 2 / Mh + 1;
+    }
+  int *bd = out + 1 + 2 * S;
+  bd[0] = 0;
+  bd[P] = S;
+  const double costPerRank = pcs[S] / P;
+  int s = 1;
+  for (int p = 1; p < P;) {
+    const double target = p * costPerRank;
+    if (pcs[s] >= target) {
+      bd[p] = (pcs[s] - target > target - pcs[s - 1]) ? s - 1 : s;
+      ++p;
+    } else {
+      ++s;
+    }
+  }
+}
+
+extern "C" int soik_vlc_plan(const int *exps, int S, int P, int k, long l, long Mh, double ratio,
+                             int *emi, int *out, double *pcs, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_vlc_plan, dim3(1), dim3(256), 0, st, exps, S, P, k, l, Mh, ratio, emi, out,
+                     pcs);
   return (int)hipGetLastError();
 }
 
@@ -697,7 +764,7 @@ extern "C" int soi_compress_noalloc(int *out, long stride_words, const double *x
   hipStream_t st = (hipStream_t)stream;
   int *emi = ptr, *nw = ptr + SOI_NOALLOC_MAX_CHUNKS;
   int e = (int)hipMemcpyAsync(emi, e_max_i, sizeof(int) * nchunk, hipMemcpyHostToDevice, st);
-  if (!e) e = soik_compress(out, stride_words, x, len, nchunk, e_max, emi, nw, st);
+  if (!e) e = soik_compress(out, stride_words, x, len, nchunk, e_max, emi, nw, nullptr, st);
   if (!e) e = (int)hipMemcpyAsync(nwords, nw, sizeof(int) * nchunk, hipMemcpyDeviceToHost, st);
   if (!e) e = (int)hipStreamSynchronize(st);
   if (e) {

@@ -184,3 +184,33 @@ def test_soi_vlc_banded_full_size(soi, orc, torch_cuda):
     # segment boundaries: a partition of the S segments over the P ranks
     assert sb[0] == 0 and sb[-1] == P and all(b <= c for b, c in zip(sb, sb[1:]))
     assert tot == max(seg) and len(cnt) == P
+
+
+@pytest.mark.parametrize("P,k,ratio", [(8, 1, 0.25), (8, 2, 3.7), (4, 4, 40.0)])
+def test_soi_vlc_async_plan_matches_oracle(soi, orc, torch_cuda, P, k, ratio):
+    """compute_soi_async with use_vlc plans on the device (no host round trip): after the
+    stream completes, the output, the cost-balanced segmentBoundaries (pfft.c:390-440, for
+    several cost ratios and k > 1) and the send counts match the oracle exactly."""
+    torch = torch_cuda
+    N = 2 ** 16
+    x = _banded(N, P * k, 13)
+    d = soi.make_descriptor(N, P, k, use_vlc=True, comm_to_comp_cost_ratio=ratio)
+    try:
+        xt = to_dev(torch, x)
+        torch.cuda.synchronize()
+        soi.compute_soi_async(d, xt.data_ptr(), None)
+        torch.cuda.synchronize()
+        got = xt.cpu().numpy()
+        sb = list(d.segmentBoundaries[: P + 1])
+        seg, tot, cnt = soi.vlc_exponents(d)
+    finally:
+        soi.free_soi_descriptor(d)
+    p = orc.params(N, P, k)
+    ref, osb = orc.compute_soi(p, x, use_vlc=True, comm_to_comp=ratio)
+    assert rel(got, ref) < 1e-12
+    assert sb == list(osb)
+    at = orc.filter_subsample(p, x)
+    oseg = [orc.max_exponent(at[s].view(np.float64)) for s in range(P * k)]
+    assert seg == oseg and tot == max(oseg)
+    sc, _ = orc.vlc_plan(P, k, at.shape[1] // P, at.shape[1], ratio, oseg)
+    assert cnt == list(sc)

@@ -77,6 +77,7 @@ void soi_internal_set_error(int status, const char *msg)
 /* ------------------------------------------------------------------------------------------ */
 #define NSTAGE 4
 #define NTSLOT 64 /* timed calls kept for soi_stage_times */
+#define SOI_MAX_FFT_GROUPS 32
 typedef struct {
   int device;
   hipStream_t stream;
@@ -120,6 +121,12 @@ typedef struct {
   int *h_vlc;
   int vlc_valid;
   hipStream_t last_stream; /* stream of the last compute call */
+  /* segment groups of the register FFT path: pass 1 of group g + 1 runs beside pass 2 of group
+   * g on a second stream, so that pass 2 reads pass 1's output while it is still in the
+   * last-level cache (fft_groups = 1: both passes over all segments, one launch each) */
+  int fft_groups;
+  hipStream_t stream2;
+  hipEvent_t gev[SOI_MAX_FFT_GROUPS + 2];
   /* timing: NSTAGE+1 events per call, in a ring of NTSLOT calls */
   int timing;
   long ncalls;          /* calls recorded since timing was enabled */
@@ -401,6 +408,9 @@ static void free_plan(soi_plan_t *pl)
   for (int c = 0; c < NTSLOT; ++c)
     for (int e = 0; e <= NSTAGE; ++e)
       if (pl->ev[c][e]) (void)hipEventDestroy(pl->ev[c][e]);
+  for (int e = 0; e < SOI_MAX_FFT_GROUPS + 2; ++e)
+    if (pl->gev[e]) (void)hipEventDestroy(pl->gev[e]);
+  if (pl->stream2) (void)hipStreamDestroy(pl->stream2);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
   free(pl);
 }
@@ -481,6 +491,18 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
   pl->f32 = d->precision == SOI_PREC_FP32;
   if (plan_filter(pl, d->window_mode) || plan_fft(pl)) goto fail_quiet;
   HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+  {
+    /* SOI_FFT_GROUPS: segment groups of the two-stream FFT schedule (a divisor of S) */
+    const char *eg = getenv("SOI_FFT_GROUPS");
+    int ng = eg ? atoi(eg) : 1;
+    if (ng < 1 || ng > SOI_MAX_FFT_GROUPS || S % ng || !pl->fft_reg) ng = 1;
+    pl->fft_groups = ng;
+    if (ng > 1) {
+      HIPCHK(hipStreamCreateWithFlags(&pl->stream2, hipStreamNonBlocking));
+      for (int e = 0; e < ng + 2; ++e)
+        HIPCHK(hipEventCreateWithFlags(&pl->gev[e], hipEventDisableTiming));
+    }
+  }
 
   /* segment boundaries p*k (pfft.c:144-147) */
   d->segmentBoundaries = (int *)malloc(sizeof(int) * (d->P + 1));
@@ -736,15 +758,35 @@ static int run_fft_f32(soi_desc_t *d, soi_plan_t *pl, soi_c32_t *X, SoiBatch b, 
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
     soi_c32_t *gt = pl->tiled ? (soi_c32_t *)pl->tbuf : NULL;
-    KCHK(soik_fft_cols_reg_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->lb1,
-                               pl->Mh, (int)pl->S, pl->Tc, pl->clay, gt, b, st));
-    mark(pl, 3, st);
-    if (gt)
-      KCHK(soik_fft_rows_tiled_f32(gt, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
-                                   (int)pl->S, b, st));
-    else
-      KCHK(soik_fft_rows_reg_f32(at, X, pl->tw2_32, winv, pl->n1, pl->n2, pl->Mh, pl->M,
-                                 (int)pl->S, pl->Tr, b, st));
+    const int ng = b.n > 0 ? 1 : pl->fft_groups, G = (int)pl->S / ng;
+    hipStream_t s2 = ng > 1 ? pl->stream2 : st;
+    if (ng > 1) {
+      HIPCHK(hipEventRecord(pl->gev[0], st));
+      HIPCHK(hipStreamWaitEvent(s2, pl->gev[0], 0));
+    }
+    for (int g = 0; g < ng; ++g) {
+      const long so = (long)g * G;
+      KCHK(soik_fft_cols_reg_f32(at + so * pl->Mh, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1,
+                                 pl->n2, pl->lb1, pl->Mh, G, pl->Tc, pl->clay,
+                                 gt ? gt + so * pl->Mh : NULL, b, st));
+      if (ng > 1) {
+        HIPCHK(hipEventRecord(pl->gev[1 + g], st));
+        HIPCHK(hipStreamWaitEvent(s2, pl->gev[1 + g], 0));
+      } else {
+        mark(pl, 3, st);
+      }
+      if (gt)
+        KCHK(soik_fft_rows_tiled_f32(gt + so * pl->Mh, X + so * pl->M, pl->tw2_32, winv, pl->n1,
+                                     pl->n2, pl->Mh, pl->M, G, b, s2));
+      else
+        KCHK(soik_fft_rows_reg_f32(at + so * pl->Mh, X + so * pl->M, pl->tw2_32, winv, pl->n1,
+                                   pl->n2, pl->Mh, pl->M, G, pl->Tr, b, s2));
+    }
+    if (ng > 1) {
+      HIPCHK(hipEventRecord(pl->gev[ng + 1], s2));
+      HIPCHK(hipStreamWaitEvent(st, pl->gev[ng + 1], 0));
+      mark(pl, 3, st);
+    }
   } else {
     KCHK(soik_fft_col_f32(at, pl->tw1_32, pl->thi32, pl->tlo32, pl->n1, pl->n2, pl->Tc, pl->B1,
                           pl->Mh, (int)pl->S, b, st));
@@ -769,15 +811,35 @@ static int run_fft(soi_desc_t *d, soi_plan_t *pl, void *Xv, SoiBatch b, hipStrea
     mark(pl, 3, st);
   } else if (pl->fft_reg) {
     soi_c64_t *gt = pl->tiled ? (soi_c64_t *)pl->tbuf : NULL;
-    KCHK(soik_fft_cols_reg(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->lb1, pl->Mh,
-                           (int)pl->S, pl->Tc, pl->clay, gt, b, st));
-    mark(pl, 3, st);
-    if (gt)
-      KCHK(soik_fft_rows_tiled(gt, X, pl->tw2, pl->winv, pl->n1, pl->n2,
-                               pl->Mh, pl->M, (int)pl->S, b, st));
-    else
-      KCHK(soik_fft_rows_reg(at, X, pl->tw2, pl->winv, pl->n1, pl->n2,
-                             pl->Mh, pl->M, (int)pl->S, pl->Tr, b, st));
+    const int ng = b.n > 0 ? 1 : pl->fft_groups, G = (int)pl->S / ng;
+    hipStream_t s2 = ng > 1 ? pl->stream2 : st;
+    if (ng > 1) {
+      HIPCHK(hipEventRecord(pl->gev[0], st));
+      HIPCHK(hipStreamWaitEvent(s2, pl->gev[0], 0));
+    }
+    for (int g = 0; g < ng; ++g) {
+      const long so = (long)g * G;
+      KCHK(soik_fft_cols_reg(at + so * pl->Mh, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2,
+                             pl->lb1, pl->Mh, G, pl->Tc, pl->clay, gt ? gt + so * pl->Mh : NULL,
+                             b, st));
+      if (ng > 1) {
+        HIPCHK(hipEventRecord(pl->gev[1 + g], st));
+        HIPCHK(hipStreamWaitEvent(s2, pl->gev[1 + g], 0));
+      } else {
+        mark(pl, 3, st);
+      }
+      if (gt)
+        KCHK(soik_fft_rows_tiled(gt + so * pl->Mh, X + so * pl->M, pl->tw2, pl->winv, pl->n1,
+                                 pl->n2, pl->Mh, pl->M, G, b, s2));
+      else
+        KCHK(soik_fft_rows_reg(at + so * pl->Mh, X + so * pl->M, pl->tw2, pl->winv, pl->n1,
+                               pl->n2, pl->Mh, pl->M, G, pl->Tr, b, s2));
+    }
+    if (ng > 1) {
+      HIPCHK(hipEventRecord(pl->gev[ng + 1], s2));
+      HIPCHK(hipStreamWaitEvent(st, pl->gev[ng + 1], 0));
+      mark(pl, 3, st);
+    }
   } else {
     KCHK(soik_fft_col(at, pl->tw1, pl->thi, pl->tlo, pl->n1, pl->n2, pl->Tc, pl->B1, pl->Mh,
                       (int)pl->S, b, st));

@@ -667,17 +667,35 @@ template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
 // with K = 80 >= BPAD + 8 covered by 20 four-deep steps.  The A operand of step ks in lane l is
 // row (m, part) = ((l % 16) >> 1, l & 1), k = l / 16, read straight from the tile rows as the
 // direct-to-LDS loads left them: 1 KB chunks with a 16-byte gap after each, so the 16 (m, part)
-// rows of one read fall on distinct banks.  Two row buffers alternate: the loads of tile t + 2
-// fill the buffer of tile t once its MFMA block is done.  The B operands (20 per lane i) stay
-// in registers.  Post phase, S-point DFT and stores as in k_filter_mfma.
+// rows of one read fall on distinct banks.  Result layout (v_mfma_f32_16x16x4_f32): D[row][col]
+// in lane col + 16 (row / 4), register row % 4, so lane col + 16g holds m = 2g (registers 0, 1:
+// real, imaginary) and m = 2g + 1 (registers 2, 3).
+//
+// Wave w of the NW = S / LPW waves owns the lanes i = NW li + w; the S-point DFT of each output
+// row splits as in the complex128 MFMA filter: LPW-point DFTs in registers, one exchange
+// through LDS, an NW-point DFT per (R, s1) whose results go straight to HBM.
+//
+// Two row buffers alternate: the loads of tile t + 2 fill tile t's buffer once its MFMA block is
+// done, and complete under tile t + 1.  The tile loop is unrolled by two so that each half names
+// its buffer statically: the compiler's wait tracking then sees that the loads in flight write
+// the other buffer, and the MFMA reads wait for nothing.  Whole tiles issue the same stores in
+// every wave, unconditionally; the transform's partial last tile runs after the loop.  The B
+// operands (20 per lane i) stay in registers.
 // ---------------------------------------------------------------------------------------------
-template <int S> struct Mfma32Cfg {
+typedef float f32x2g __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) f32x2g gf32x2;
+
+__device__ __forceinline__ void store_nt_global(gf32x2 *p, float2 v)
+{
+  __builtin_nontemporal_store(f32x2g{v.x, v.y}, p);
+}
+
+template <int S, int LPW> struct Mfma32Cfg {
   static constexpr int EPL = 2;                 // complex64 values per 16-byte direct-to-LDS load
   static constexpr int TJ = 24;                 // row groups per tile: 8 MFMA rows x 3 shifts
   static constexpr int NK = 20;                 // four-deep K steps
   static constexpr int K = 4 * NK;
   static constexpr int ROWS = 12 * 7 + K;       // input rows read per tile
-  static constexpr int LPW = S < 2 ? S : 2;    // lanes per wave (16 waves at S = 32)
   static constexpr int NWAVES = S / LPW;
   static constexpr int T = NWAVES * 64;
   static constexpr int CH = 64 * EPL;           // values per wave-wide load (one chunk)
@@ -685,34 +703,40 @@ template <int S> struct Mfma32Cfg {
   static constexpr int NDMA = (ROWS * S + CH - 1) / CH;  // chunks per tile
   static constexpr int RAWN = NDMA * CP;
   static constexpr int GR = TJ * 5;             // output rows R per tile
-  static constexpr int GS = S + 1;
-  static constexpr int NSC = (GR * S + T - 1) / T;
-  static constexpr int NSC_MIN = (GR * S) / T;
+  static constexpr int GP = S + 1;              // exchange row pitch: [R][s1 * NWAVES + w]
+  static constexpr int NTASK = GR * LPW;
+  static constexpr int NC = (NTASK + T - 1) / T;
+  static constexpr int DUPD = NC * T - NTASK;
+  static constexpr bool DUP = DUPD < 64;
+  static_assert(LPW == 2 || LPW == 4 || LPW == 8, "in-register DFT size");
 };
 
-template <int S>
-__global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filter_mfma32(
+constexpr int mfma32_threads(int S, int LPW) { return S / LPW * 64; }
+
+template <int S, int LPW>
+__global__ __launch_bounds__(mfma32_threads(S, LPW), mfma32_threads(S, LPW) / 256) void k_filter_mfma32(
     const float2 *__restrict__ x, float2 *__restrict__ at, const float *__restrict__ wt,
     const float2 *__restrict__ post, const float2 *__restrict__ twS, long M, long Mh, long nJ,
     int tiles_per_block, SoiBatch b)
 {
-  typedef Mfma32Cfg<S> CF;
+  typedef Mfma32Cfg<S, LPW> CF;
   x = batch_ptr(x, b.x);
   at = batch_ptr(at, b.at);
-  constexpr int LPW = CF::LPW, NK = CF::NK, CH = CF::CH, CP = CF::CP;
+  constexpr int NK = CF::NK, CH = CF::CH, CP = CF::CP, NW = CF::NWAVES, GP = CF::GP;
+  gf32x2 *const atg = (gf32x2 *)at;
   __shared__ __align__(16) float2 raw0[CF::RAWN];
   __shared__ __align__(16) float2 raw1[CF::RAWN];
-  __shared__ __align__(16) float2 G[CF::GR * CF::GS];
+  __shared__ __align__(16) float2 G[CF::GR * GP];
   __shared__ __align__(16) float2 tw[S];
   __shared__ __align__(16) float2 pfl[5 * S];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int n = lane & 15, g = lane >> 4;
 
-  float wb[LPW][NK];  // B operands of this wave's lanes
+  float wb[LPW][NK];  // B operands of this wave's lanes i = NW li + wave
 #pragma unroll
   for (int li = 0; li < LPW; ++li)
 #pragma unroll
-    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((wave * LPW + li) * NK + ks) * 64 + lane];
+    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((li * NW + wave) * NK + ks) * 64 + lane];
   for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
   for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
 
@@ -725,7 +749,7 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
   // the NDMA chunks of a tile's rows (contiguous from element g0, wrapping once at N)
   auto dma = [&](float2 *dst, long g0) {
 #pragma unroll
-    for (int c = wave; c < CF::NDMA; c += CF::NWAVES) {
+    for (int c = wave; c < CF::NDMA; c += NW) {
       long gi = g0 + c * CH + lane * CF::EPL;  // N and g0 are multiples of EPL
       if (gi >= N) gi -= N;
       __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * CP), 16, 0, 0);
@@ -737,61 +761,115 @@ __global__ __launch_bounds__(Mfma32Cfg<S>::T, Mfma32Cfg<S>::T / 256) void k_filt
   lds_barrier();
   if (t0 + 1 < t1) dma(raw1, (t0 + 1) * CF::TJ * 4 * S);
 
+  // A operand of lane (n, g): row (m, part) = (n >> 1, n & 1), k = g; value index within the
+  // tile (12 m + 4 ks + g) * S + i with i = NW li + wave
   const int am = n >> 1, apart = n & 1;
-  for (long tile = t0; tile < t1; ++tile) {
-    const bool more = tile + 1 < t1;
-    float2 *cur = ((tile - t0) & 1) ? raw1 : raw0;
+  const int v0 = (12 * am + g) * S + wave;
+
+  // One tile from buffer cur (B = 0: raw0, 1: raw1).  PART: the transform's partial last tile.
+  auto body = [&](long tile, auto buf_tag, auto part_tag) {
+    constexpr int BUF = decltype(buf_tag)::value;
+    constexpr bool PART = decltype(part_tag)::value;
+    float2 *cur = BUF ? raw1 : raw0;
     const float *curf = reinterpret_cast<const float *>(cur);
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+    const int ln = tv & 63, nv = ln & 15, gv = ln >> 4;
 
     // ---- MFMA: one accumulation chain per lane i, all on v_mfma_f32_16x16x4_f32
     f32x4 acc[LPW];
 #pragma unroll
     for (int li = 0; li < LPW; ++li) acc[li] = f32x4{0, 0, 0, 0};
+    // A operand offsets from an opaque copy of the lane's first value index: computed once per
+    // kernel they would be loop-invariant and hoisted (80 live registers).  When a K step is a
+    // whole number of chunks (4S a multiple of CH) and the lanes i of a wave share a chunk, a
+    // read is the lane's base plus a compile-time offset.
+    int vb = v0;
+    asm volatile("" : "+v"(vb));
+    constexpr bool FAST = (4 * S) % CH == 0;
+    const int abase = 2 * ((vb / CH) * CP + vb % CH) + apart;
 #pragma unroll
     for (int ks = 0; ks < NK; ++ks) {
 #pragma unroll
       for (int li = 0; li < LPW; ++li) {
-        const int e = (12 * am + 4 * ks + g) * S + wave * LPW + li;  // row * S + lane i
-        const float a = curf[2 * ((e / CH) * CP + e % CH) + apart];
+        float a;
+        if constexpr (FAST) {
+          a = curf[abase + 2 * (ks * (4 * S / CH) * CP + li * NW)];
+        } else {
+          const int e = vb + 4 * ks * S + li * NW;
+          a = curf[2 * ((e / CH) * CP + e % CH) + apart];
+        }
         acc[li] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wb[li][ks], acc[li], 0, 0, 0);
       }
     }
-    lds_barrier();  // every wave is done reading cur (and the previous tile's G)
+    lds_barrier();  // every wave is done reading cur and the previous tile's G
 
-    // ---- post phase into G: accumulator j of this lane is part j & 1 of row m = 2g + (j >> 1)
-    if (n < 15) {
+    // ---- post phase and the LPW-point DFTs over this wave's lanes i, twiddled by w_S^{w s1},
+    // into the exchange rows G[R][s1 * NW + w]; column nv = theta + 5D (15 is unused)
 #pragma unroll
-      for (int li = 0; li < LPW; ++li) {
-        const int i = wave * LPW + li;
-        const float2 pf = pfl[(n % 5) * S + i];
+    for (int h = 0; h < 2; ++h) {
+      float2 v[LPW];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          G[(15 * (2 * g + h) + n) * CF::GS + i] = cmul(float2{acc[li][2 * h], acc[li][2 * h + 1]}, pf);
+      for (int li = 0; li < LPW; ++li)
+        v[li] = cmul(float2{acc[li][2 * h], acc[li][2 * h + 1]}, pfl[(nv % 5) * S + li * NW + wave]);
+      dft_r<LPW>(v);
+      if (nv < 15) {
+        float2 *gr = G + (15 * (2 * gv + h) + nv) * GP + wave;
+        gr[0] = v[0];
+#pragma unroll
+        for (int s1 = 1; s1 < LPW; ++s1) gr[s1 * NW] = cmul(v[s1], tw[(wave * s1) % S]);
       }
     }
-    if (more) {
+    if constexpr (!PART) {
       // the next tile's rows are complete once at most this wave's stores of the previous tile
-      // are outstanding (they were issued after the rows; none before the first tile)
+      // are outstanding (issued after those rows; none before the block's first tile).
+      // Unconditional in whole tiles (harmless when no tile follows), so that every path back to
+      // the loop head has waited.
       asm volatile("" ::: "memory");
       if (tile == t0) wait_vmcnt<0>();
-      else wait_vmcnt<CF::NSC_MIN>();
+      else if constexpr (CF::DUP) wait_vmcnt<CF::NC * NW>();
+      else wait_vmcnt<0>();
     }
     lds_barrier();  // publishes G and the next tile's rows; cur is free
-    if (tile + 2 < t1) dma(cur, (tile + 2) * CF::TJ * 4 * S);
+    if (!PART && tile + 2 < t1) dma(cur, (tile + 2) * CF::TJ * 4 * S);
 
-    // ---- S-point DFT of each row
-    sdft_passes<S, CF::GR, CF::GS, CF::T, 0, 1, float2>(G, tw, tid);
-
-    // ---- store segment-major: R = 5 J0 + r (runs of GR consecutive R per segment)
+    // ---- NW-point DFT over the waves for each (R, s1), stored segment-major
     const long R0 = tile * CF::TJ * 5;
 #pragma unroll
-    for (int c = 0; c < CF::NSC; ++c) {
-      const int e = tid + c * CF::T;
-      if ((CF::GR * S) % CF::T == 0 || e < CF::GR * S) {
-        const int s = e / CF::GR, r = e % CF::GR;
-        if (tile * CF::TJ + r / 5 < nJ) store_nt(at + (long)s * Mh + R0 + r, G[r * CF::GS + s]);
+    for (int c = 0; c < CF::NC; ++c) {
+      int e = tv + c * CF::T;
+      if constexpr (CF::DUP) {
+        if (e >= CF::NTASK) e -= CF::DUPD;
+      }
+      if (CF::DUP || CF::NTASK % CF::T == 0 || e < CF::NTASK) {
+        const int r = e % CF::GR, s1 = e / CF::GR;
+        float2 u[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) u[w] = G[r * GP + s1 * NW + w];
+        dft_r<NW>(u);
+        if (!PART || tile * CF::TJ + r / 5 < nJ) {
+#pragma unroll
+          for (int s2 = 0; s2 < NW; ++s2)
+            store_nt_global(atg + (long)(s1 + LPW * s2) * Mh + R0 + r, u[s2]);
+        }
       }
     }
+  };
+
+  // whole tiles, two per iteration (raw0 then raw1); the partial last tile after them
+  const long tfull = min(t1, nJ / CF::TJ);
+  long tile = t0;
+  for (; tile + 1 < tfull; tile += 2) {
+    body(tile, std::integral_constant<int, 0>{}, std::false_type{});
+    body(tile + 1, std::integral_constant<int, 1>{}, std::false_type{});
+  }
+  if (tile < tfull) {
+    body(tile, std::integral_constant<int, 0>{}, std::false_type{});
+    ++tile;
+  }
+  if (tile < t1) {
+    if ((tile - t0) & 1) body(tile, std::integral_constant<int, 1>{}, std::true_type{});
+    else body(tile, std::integral_constant<int, 0>{}, std::true_type{});
   }
 }
 
@@ -1110,29 +1188,29 @@ __global__ void k_filter_mfma32_table(const double *__restrict__ taps, float *__
   }
 }
 
-template <int S> static int launch_mfma32(const SoiFilterArgs *a, hipStream_t st)
+template <int S, int LPW> static int launch_mfma32(const SoiFilterArgs *a, hipStream_t st)
 {
-  typedef Mfma32Cfg<S> CF;
+  typedef Mfma32Cfg<S, LPW> CF;
   if ((long)CF::NDMA * CF::CH > a->M * S) return (int)hipErrorInvalidValue;  // wraps at most once
   const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
   long nb = 256;
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL(k_filter_mfma32<S>, dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1), dim3(CF::T), 0,
-                     st, (const float2 *)a->x, (float2 *)a->at, (const float *)a->wt,
-                     (const float2 *)a->post, (const float2 *)a->twS, a->M, a->Mh, a->nJ, tpb,
-                     a->b);
+  hipLaunchKernelGGL((k_filter_mfma32<S, LPW>), dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1),
+                     dim3(CF::T), 0, st, (const float2 *)a->x, (float2 *)a->at,
+                     (const float *)a->wt, (const float2 *)a->post, (const float2 *)a->twS, a->M,
+                     a->Mh, a->nJ, tpb, a->b);
   return (int)hipGetLastError();
 }
 
 static int filter_mfma32(const SoiFilterArgs *a, hipStream_t st)
 {
   switch (a->S) {
-    case 4: return launch_mfma32<4>(a, st);
-    case 8: return launch_mfma32<8>(a, st);
-    case 16: return launch_mfma32<16>(a, st);
-    case 32: return launch_mfma32<32>(a, st);
+    case 4: return launch_mfma32<4, 2>(a, st);
+    case 8: return launch_mfma32<8, 2>(a, st);
+    case 16: return launch_mfma32<16, 2>(a, st);
+    case 32: return launch_mfma32<32, 4>(a, st);
     default: return (int)hipErrorInvalidValue;
   }
 }
@@ -1141,7 +1219,7 @@ static int filter_mfma32(const SoiFilterArgs *a, hipStream_t st)
 // most 88 (split-bf16, mode 1) or 72 (FP32, mode 2) padded taps.
 extern "C" int soik_filter_mfma_supported(int S, int n_mu, int d_mu, int ct, int BPAD, int mode)
 {
-  const int maxb = mode == 1 ? 88 : Mfma32Cfg<4>::K - 8;
+  const int maxb = mode == 1 ? 88 : Mfma32Cfg<4, 2>::K - 8;
   return (mode == 1 || mode == 2 || mode == 3) && !ct && n_mu == 5 && d_mu == 4 && BPAD <= maxb &&
          (S == 4 || S == 8 || S == 16 || S == 32);
 }
@@ -1150,7 +1228,7 @@ extern "C" size_t soik_filter_mfma_table_bytes(int S, int BPAD, int mode)
 {
   (void)BPAD;
   if (mode == 3) return (size_t)S * Mfma64Cfg<4, 2>::NK * 64 * sizeof(double);
-  if (mode == 2) return (size_t)S * Mfma32Cfg<4>::NK * 64 * sizeof(float);
+  if (mode == 2) return (size_t)S * Mfma32Cfg<4, 2>::NK * 64 * sizeof(float);
   return (size_t)S * MfmaCfg<4>::NCH * 3 * 64 * 16;
 }
 
@@ -1162,7 +1240,7 @@ extern "C" int soik_filter_mfma_table(const double *taps, void *wt, int S, int B
                        BPAD, Mfma64Cfg<4, 2>::NK);
   else if (mode == 2)
     hipLaunchKernelGGL(k_filter_mfma32_table, dim3(64), dim3(256), 0, st, taps, (float *)wt, S,
-                       BPAD, Mfma32Cfg<4>::NK);
+                       BPAD, Mfma32Cfg<4, 2>::NK);
   else
     hipLaunchKernelGGL(k_filter_mfma_table, dim3(64), dim3(256), 0, st, taps, (unsigned short *)wt,
                        S, BPAD, MfmaCfg<4>::NCH);

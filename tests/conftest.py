@@ -38,3 +38,26 @@ def torch_cuda():
     import torch
     assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
     return torch
+
+
+def pytest_terminal_summary(terminalreporter):
+    """Achieved relative-L2 errors of the GPU parity comparisons (the largest per test, in
+    increasing order), printed and written to gpurun_out/parity_errors.json."""
+    gu = sys.modules.get("gpu_util")
+    rec = gu.recorded() if gu else {}
+    if not rec:
+        return
+    import json
+    rows = sorted(((max(v), k, len(v)) for k, v in rec.items()), key=lambda t: t[0])
+    terminalreporter.write_line("achieved relative L2 errors (largest per test; each comparison "
+                                "in order when there are at most 4):")
+    for e, k, n in rows:
+        each = "  [" + ", ".join(f"{v:.2e}" for v in rec[k]) + "]" if 1 < n <= 4 else f"  ({n})"
+        terminalreporter.write_line(f"  {e:.2e}  {k.split('::', 1)[-1]}{each}")
+    out = os.path.join(ROOT, "gpurun_out")
+    try:
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, "parity_errors.json"), "w") as f:
+            json.dump({k: v for e, k, n in rows for v in [rec[k]]}, f, indent=1)
+    except OSError:
+        pass

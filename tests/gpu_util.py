@@ -1,5 +1,8 @@
-"""Helpers for GPU tests: device memory via torch (plumbing), raw copies via the HIP runtime."""
+"""Helpers for GPU tests: device memory via torch (plumbing), raw copies via the HIP runtime,
+and the relative-L2 helper, which records every achieved error under the running test's id so
+that the session ends with a table of them (conftest.pytest_terminal_summary)."""
 import ctypes
+import os
 
 import numpy as np
 
@@ -27,10 +30,21 @@ def to_dev(torch, x):
     return torch.from_numpy(np.ascontiguousarray(x)).cuda()
 
 
+_RECORD = {}
+
+
 def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    r = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    tid = os.environ.get("PYTEST_CURRENT_TEST", "?").rsplit(" (", 1)[0]
+    _RECORD.setdefault(tid, []).append(r)
+    return r
+
+
+def recorded():
+    """{test id: [relative L2 errors of its comparisons, in order]}"""
+    return _RECORD
 
 
 def hip_malloc(nbytes):

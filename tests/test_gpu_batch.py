@@ -99,3 +99,55 @@ def test_autotune_keeps_results(soi, orc, torch_cuda):
         assert 0 <= best < nv
     finally:
         soi.free_soi_descriptor(d)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_512_member_workload(soi, orc, torch_cuda, prec):
+    """The batching workload at its stated size: the exact 512-member draw bench.py times
+    (seed 2028, inputs seeded 7000 + i), captured with autotuning and replayed twice.  Every
+    member must equal its eager compute_soi bit for bit after each replay, and one member of
+    each distinct (N, P) is checked against the oracle at the form's bar (1e-12 FP64, 1e-6 FP32
+    against the FP64 transform of the same input)."""
+    torch = torch_cuda
+    shapes = draw_shapes(np.random.default_rng(2028), 512)
+    assert len(set(shapes)) == 14
+    print(f"case 512-member batch prec={prec}", flush=True)
+    cdt = torch.complex64 if prec else torch.complex128
+    descs, bufs, inputs = [], [], []
+    batch = None
+    try:
+        for i, (N, P) in enumerate(shapes):
+            descs.append(soi.make_descriptor(N, P, precision=prec))
+            x = torch.empty(N, dtype=torch.complex128, device="cuda")
+            assert soi.lib().soi_fill_uniform(x.data_ptr(), N, 7000 + i, None) == 0
+            x = x.to(cdt)
+            inputs.append(x.clone())
+            bufs.append(x)
+        torch.cuda.synchronize()
+        batch = soi.Batch(descs, [b.data_ptr() for b in bufs], autotune=True)
+        for rep in range(2):
+            for b, x0 in zip(bufs, inputs):
+                b.copy_(x0)
+            torch.cuda.synchronize()
+            batch.run()
+            torch.cuda.synchronize()
+            for i, (d, x0) in enumerate(zip(descs, inputs)):
+                y = x0.clone()
+                torch.cuda.synchronize()
+                soi.compute_soi(d, y.data_ptr())
+                assert torch.equal(bufs[i], y), f"replay {rep}, member {i} {shapes[i]}"
+        seen = set()
+        for i, (N, P) in enumerate(shapes):
+            if (N, P) in seen:
+                continue
+            seen.add((N, P))
+            x = inputs[i].to(torch.complex128).cpu().numpy()
+            ref, _ = orc.compute_soi(orc.params(N, P), x)
+            err = rel(bufs[i].to(torch.complex128).cpu().numpy(), ref)
+            assert err < (1e-6 if prec else 1e-12), (i, (N, P), err)
+        assert len(seen) == 14
+    finally:
+        if batch is not None:
+            batch.close()
+        for d in descs:
+            soi.free_soi_descriptor(d)

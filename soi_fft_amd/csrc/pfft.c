@@ -307,10 +307,13 @@ static int plan_fft(soi_plan_t *pl)
       }
       /* column-pass kernel variant: 1 = one padded LDS run per sequence (8-wide tiles of 2048
        * or 2560); 2 = one 4-wide block per CU with the whole register budget (2560, the
-       * default there: 20 values per thread do not fit 1024-thread blocks); SOI_FFT_CVAR
-       * overrides for tuning */
+       * complex128 default there: 20 values per thread do not fit 1024-thread blocks).
+       * Complex64 2560-point columns default to 4-wide tiles in two exchange groups, two
+       * blocks per CU (their whole-complex exchanges fit in 44 KB).  SOI_FFT_CVAR overrides
+       * for tuning */
       const char *el = getenv("SOI_FFT_CVAR");
-      int cv = el && !stile ? atoi(el) : (pl->n1 == 2560 ? 2 : 0);
+      int cv = el && !stile ? atoi(el) : (pl->n1 == 2560 && !pl->f32 ? 2 : 0);
+      if (!el && !ec && pl->n1 == 2560 && pl->f32 && pl->n2 % 4 == 0) pl->Tc = 4;
       pl->clay = 0;
       if (cv == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) pl->clay = 1;
       if (cv == 2 && pl->n1 == 2560) {

@@ -116,29 +116,74 @@ template <int... Rs> struct Radices {
 //             waves that hold consecutive positions of one sequence (the row pass); the LP offset
 //             keeps the T-sequence transpose reads of the row pass on distinct banks
 template <int L, int TG> struct QFast {
+  static constexpr bool full = false;  // complex128: exchange in real / imaginary rounds
   static constexpr int size = (L + L / 16) * TG;
   static __device__ __forceinline__ int at(int pos, int q) { return (pos + (pos >> 4)) * TG + q; }
 };
 template <int L, int TG> struct QFast8 {
+  static constexpr bool full = false;
   static constexpr int size = (L + L / 8) * TG;
   static __device__ __forceinline__ int at(int pos, int q) { return (pos + (pos >> 3)) * TG + q; }
 };
 template <int L, int TG> struct SeqMajor {
+  static constexpr bool full = false;
   static constexpr int LP0 = L + L / 16;
   static constexpr int LP = LP0 + ((2 - LP0 % 16) + 16) % 16;
   static constexpr int size = LP * TG;
   static __device__ __forceinline__ int at(int pos, int q) { return q * LP + pos + (pos >> 4); }
 };
 
+// Complex64 exchanges move whole complex values in one round when the layout fits in LDS: half
+// the LDS instructions and barriers of the real / imaginary rounds.  (Complex128 tiles keep the
+// rounds: exchanging them whole in two groups of the T sequences, the only way they fit,
+// measured slower for the 2560-point columns: 1.83 against 1.60 ms at N = 2^27.)  A layout may
+// set `full` to ask for whole exchanges of complex128 values as well.
+template <typename LY, typename C> constexpr bool exch_full()
+{
+  return (sizeof(real_t<C>) == 4 || LY::full) && (size_t)LY::size * sizeof(C) <= 160 * 1024;
+}
+template <typename LY, typename C> constexpr size_t exch_lds_bytes()
+{
+  return (size_t)LY::size * (exch_full<LY, C>() ? sizeof(C) : sizeof(real_t<C>));
+}
+
 // Exchange a thread's stage outputs through LDS into the next stage's input order.  Outputs of
 // stage s (radix R, span Ns) for butterfly j land at (j - j%Ns)*R + j%Ns + k*Ns; stage s+1
-// (radix R2) butterfly j2 reads j2 + k*L/R2.  Done in parts: real then imaginary halves, and
-// NQP groups of the T sequences, so the LDS footprint is L*(T/NQP)*8 bytes (plus padding).
+// (radix R2) butterfly j2 reads j2 + k*L/R2.  Done in parts: real then imaginary halves (or whole
+// complex values, exch_full), and NQP groups of the T sequences, so the LDS footprint is
+// L*(T/NQP)*8 bytes (plus padding).
 template <int L, int E, int T, int NQP, int TS, int R, int NS, int R2, typename LY, typename C>
 __device__ __forceinline__ void exchange(C (&v)[E], real_t<C> *lds, int q, int ts)
 {
   constexpr int NB = E / R, NB2 = E / R2, TG = T / NQP;
   const int qg = q / TG, ql = q % TG;
+  if constexpr (exch_full<LY, C>()) {
+    C *ldc = reinterpret_cast<C *>(lds);
+#pragma unroll
+    for (int g = 0; g < NQP; ++g) {
+      if (qg == g) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          const int j = ts + TS * c;
+          const int jm = j % NS;
+          const int base = (j - jm) * R + jm;
+#pragma unroll
+          for (int k = 0; k < R; ++k) ldc[LY::at(base + k * NS, ql)] = v[c * R + k];
+        }
+      }
+      lds_barrier();
+      if (qg == g) {
+#pragma unroll
+        for (int c = 0; c < NB2; ++c) {
+          const int j = ts + TS * c;
+#pragma unroll
+          for (int k = 0; k < R2; ++k) v[c * R2 + k] = ldc[LY::at(j + k * (L / R2), ql)];
+        }
+      }
+      lds_barrier();
+    }
+    return;
+  }
 #pragma unroll
   for (int g = 0; g < NQP; ++g) {
 #pragma unroll
@@ -447,7 +492,7 @@ static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
 {
   if (tout) {
     auto kt = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, true>;
-    constexpr size_t lds = (size_t)LY::size * sizeof(real_t<C>);
+    constexpr size_t lds = exch_lds_bytes<LY, C>();
     static bool once_t = false;
     if (!once_t) {
       (void)hipFuncSetAttribute((const void *)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -458,7 +503,7 @@ static int launch_cols(C *data, const C *tw, const C *thi, const C *tlo,
     return (int)hipGetLastError();
   }
   auto kern = k_fft_cols<L, E, T, NQP, BPC, RS, LY, C, false>;
-  constexpr size_t lds = (size_t)LY::size * sizeof(real_t<C>);
+  constexpr size_t lds = exch_lds_bytes<LY, C>();
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -475,7 +520,7 @@ static int launch_rows(const C *data, C *X, const C *tw, const double2 *winv,
                        int n1, long Mh, long M, int S, SoiBatch b, hipStream_t st)
 {
   auto kern = k_fft_rows<L, E, T, NQP, BPC, RS, C>;
-  constexpr size_t lds = (size_t)SeqMajor<L, T / NQP>::size * sizeof(real_t<C>);
+  constexpr size_t lds = exch_lds_bytes<SeqMajor<L, T / NQP>, C>();
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {
@@ -492,7 +537,7 @@ static int launch_rows_t(const C *data, C *X, const C *tw, const double2 *winv, 
                          long M, int S, SoiBatch b, hipStream_t st)
 {
   auto kern = k_fft_rows_t<L, E, BPC, RS, C>;
-  constexpr size_t lds = (size_t)QFast<L, 8>::size * sizeof(real_t<C>);
+  constexpr size_t lds = exch_lds_bytes<QFast<L, 8>, C>();
   static_assert(lds <= 160 * 1024, "LDS");
   static bool once = false;
   if (!once) {

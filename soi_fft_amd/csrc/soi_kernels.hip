@@ -603,8 +603,12 @@ extern "C" int soik_compress(int *out, long stride, const double *x, int len, in
 {
   if (nchunk <= 0) return 0;
   long maxw = (long)len * 56 / 32 / 1 + 64;
+  // blocks per chunk: enough for about 2048 blocks in all (the kernel strides over the chunk's
+  // words), not one thread per possible word: a chunk of short fields has few words, and a
+  // grid sized for 55-bit fields would be mostly idle blocks
   unsigned gx = nblk(maxw, 256);
-  if (gx > 4096) gx = 4096;
+  const unsigned cap = (unsigned)((2048 + nchunk - 1) / nchunk);
+  if (gx > cap) gx = cap;
   hipLaunchKernelGGL(k_compress, dim3(gx, nchunk), dim3(256), 0, st, out, stride, x, len, nchunk,
                      e_max, emi, nwords, e_max_dev);
   return (int)hipGetLastError();
@@ -615,7 +619,8 @@ extern "C" int soik_decompress(double *out, const int *in, long stride, int len,
 {
   if (nchunk <= 0) return 0;
   unsigned gx = nblk(len, 256);
-  if (gx > 4096) gx = 4096;
+  const unsigned cap = (unsigned)((2048 + nchunk - 1) / nchunk);  // about 2048 blocks in all
+  if (gx > cap) gx = cap;
   hipLaunchKernelGGL(k_decompress, dim3(gx, nchunk), dim3(256), 0, st, out, in, stride, len,
                      nchunk, e_max, emi, e_max_dev);
   return (int)hipGetLastError();
@@ -631,28 +636,35 @@ extern "C" int soik_decompress(double *out, const int *in, long stride, int len,
 //      [1+2S .. 1+2S+P] segment boundaries; pcs: S + 1 doubles of scratch.
 __global__ void k_vlc_plan(const int *__restrict__ exps, int S, int P, int k, long l, long Mh,
                            double ratio, int *__restrict__ emi, int *__restrict__ out,
-                           double *__restrict__ pcs)
+                           double *__restrict__ pcs_g)
 {
 #pragma clang fp contract(off)
-  __shared__ int smax;
+  // segment maxima and prefix costs in LDS when S is small enough (the serial walk below then
+  // reads no global memory), else in the caller's scratch
+  constexpr int SMAX = 2048;
+  __shared__ int smax, segs[SMAX];
+  __shared__ double pcs_s[SMAX + 1];
+  const bool inl = S <= SMAX;
+  double *pcs = inl ? pcs_s : pcs_g;
   if (threadIdx.x == 0) smax = INT_MIN;
   __syncthreads();
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
     int m = INT_MIN;
     for (int p = 0; p < P; ++p) m = max(m, exps[s * P + p]);
     out[1 + s] = m;
+    if (inl) segs[s] = m;
     atomicMax(&smax, m);
   }
   __syncthreads();
   const int total = smax;
-  for (int c = threadIdx.x; c < S * P; c += blockDim.x) emi[c] = out[1 + c / P];
+  for (int c = threadIdx.x; c < S * P; c += blockDim.x) emi[c] = inl ? segs[c / P] : out[1 + c / P];
   if (threadIdx.x != 0) return;
   out[0] = total;
   pcs[0] = 0;
   for (int p = 0; p < P; ++p)
     for (int ik = 0; ik < k; ++ik) {
       const int s = p * k + ik;
-      int bits = 52 + 3 - (total - out[1 + s]);
+      int bits = 52 + 3 - (total - (inl ? segs[s] : out[1 + s]));
       if (bits > 31) ++bits;
       const int lenPerSimdLane = (int)((l * 2 / 16 * bits + 31) / 32);
       const int remainder = (int)((l * 2 % 16 * bits + 31) / 32);

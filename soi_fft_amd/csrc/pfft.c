@@ -118,6 +118,7 @@ typedef struct {
    * after each run, in pinned host memory (the descriptor's segmentBoundaries points into it) */
   int *d_exps, *d_emi, *d_nwords, *d_vlc;
   double *d_pcs;
+  long *d_upre; /* exclusive prefix of the chunks' work units (flat compress) */
   int *h_vlc;
   int vlc_valid;
   hipStream_t last_stream; /* stream of the last compute call */
@@ -407,6 +408,7 @@ static void free_plan(soi_plan_t *pl)
   (void)hipFree(pl->d_nwords);
   (void)hipFree(pl->d_vlc);
   (void)hipFree(pl->d_pcs);
+  (void)hipFree(pl->d_upre);
   if (pl->h_vlc) (void)hipHostFree(pl->h_vlc);
   for (int c = 0; c < NTSLOT; ++c)
     for (int e = 0; e <= NSTAGE; ++e)
@@ -642,6 +644,7 @@ void init_soi_descriptor(soi_desc_t *d, MPI_Comm comm, cfft_size_t k)
       HIPCHK(hipMalloc((void **)&pl->d_nwords, nch * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_vlc, nvlc * sizeof(int)));
       HIPCHK(hipMalloc((void **)&pl->d_pcs, (size_t)(S + 1) * sizeof(double)));
+      HIPCHK(hipMalloc((void **)&pl->d_upre, (size_t)(nch + 1) * sizeof(long)));
       HIPCHK(hipHostMalloc((void **)&pl->h_vlc, nvlc * sizeof(int), hipHostMallocDefault));
       /* segmentBoundaries lives in the pinned plan copy, so that the run's device plan reaches
        * it by an asynchronous copy */
@@ -738,10 +741,10 @@ static int run_vlc(soi_desc_t *d, soi_plan_t *pl, hipStream_t st)
    * with Mh = P*l the chunks are contiguous in (s, p) order, stride 2l doubles */
   KCHK(soik_max_exponent(at, (int)(2 * l), nch, pl->d_exps, st));
   KCHK(soik_vlc_plan(pl->d_exps, S, P, pl->k, l, pl->Mh, d->comm_to_comp_cost_ratio, pl->d_emi,
-                     pl->d_vlc, pl->d_pcs, st));
-  KCHK(soik_compress(pl->delta, 4 * l, at, (int)(2 * l), nch, 0, pl->d_emi, pl->d_nwords,
-                     pl->d_vlc, st));
-  KCHK(soik_decompress(at, pl->delta, 4 * l, (int)(2 * l), nch, 0, pl->d_emi, pl->d_vlc, st));
+                     pl->d_vlc, pl->d_pcs, pl->d_upre, pl->d_nwords, st));
+  KCHK(soik_compress_flat(pl->delta, 4 * l, at, (int)(2 * l), nch, pl->d_vlc, pl->d_emi,
+                          pl->d_upre, st));
+  KCHK(soik_decompress_flat(at, pl->delta, 4 * l, (int)(2 * l), nch, pl->d_vlc, pl->d_emi, st));
   HIPCHK(hipMemcpyAsync(pl->h_vlc, pl->d_vlc, sizeof(int) * (size_t)(1 + 2 * S + P + 1),
                         hipMemcpyDeviceToHost, st));
   pl->vlc_valid = 1;

@@ -99,7 +99,11 @@ int soik_decompress(double *out, const int *in, long stride, int len, int nchunk
                     const int *emi, const int *e_max_dev, hipStream_t st);
 /* VLC planning on the device: out = [total, seg_max[S], send_counts[S], boundaries[P+1]] */
 int soik_vlc_plan(const int *exps, int S, int P, int k, long l, long Mh, double ratio, int *emi,
-                  int *out, double *pcs, hipStream_t st);
+                  int *out, double *pcs, long *upre, int *nwords, hipStream_t st);
+int soik_compress_flat(int *out, long stride, const double *x, int len, int nchunk,
+                       const int *e_max_dev, const int *emi, const long *upre, hipStream_t st);
+int soik_decompress_flat(double *out, const int *in, long stride, int len, int nchunk,
+                         const int *e_max_dev, const int *emi, hipStream_t st);
 int soik_gen_uniform(soi_c64_t *x, long N, uint64_t seed, hipStream_t st);
 int soik_gen_tones(soi_c64_t *x, long N, const double *tf, const double *tph, int ntones,
                    double amp, double noise_sigma, uint64_t seed, hipStream_t st);

@@ -308,8 +308,14 @@ template <typename F>
 __device__ __forceinline__ uint32_t vlc_word(long pos, int fbits, long nfields, F field)
 {
   uint32_t word = 0;
-  long f0 = pos / fbits;
-  long f1 = (pos + 31) / fbits;
+  long f0, f1;
+  if (pos + 31 <= (long)UINT32_MAX) {  // 32-bit divisions: a fraction of the 64-bit sequence
+    f0 = (uint32_t)pos / (uint32_t)fbits;
+    f1 = (uint32_t)(pos + 31) / (uint32_t)fbits;
+  } else {
+    f0 = pos / fbits;
+    f1 = (pos + 31) / fbits;
+  }
   for (long f = f0; f <= f1 && f < nfields; ++f) {
     uint64_t val = field(f);
     long fstart = f * fbits;
@@ -324,8 +330,79 @@ __device__ __forceinline__ uint32_t vlc_word(long pos, int fbits, long nfields, 
   return word;
 }
 
-// compress: one thread per output word.  Stream c: vector rows (16 lanes interleaved) then the
-// scalar tail.  nwords[c] = compress()'s return value.
+// Word wi of a chunk's stream (compress.c:205-372): vector rows (16 lanes interleaved) then the
+// scalar tail.
+__device__ __forceinline__ uint32_t compress_word(const double *__restrict__ in, int len,
+                                                  const VlcParams &v, long wi)
+{
+  const long nvf = len / 16;                 // fields per lane in the vector body
+  const long vrows = (nvf * v.fbits + 31) / 32;
+  if (wi < vrows * 16) {
+    const long row = wi / 16;
+    const int lane = (int)(wi % 16);
+    return vlc_word(row * 32, v.fbits, nvf,
+                    [&](long f) { return vlc_encode(in[f * 16 + lane], v, true); });
+  }
+  const long ntail = len - nvf * 16;
+  const double *tin = in + nvf * 16;
+  return vlc_word((wi - vrows * 16) * 32, v.fbits, ntail,
+                  [&](long f) { return vlc_encode(tin[f], v, false); });
+}
+
+// Words of a chunk's stream (compress()'s return value)
+__device__ __forceinline__ long compress_words(int len, const VlcParams &v)
+{
+  if (v.i2Len <= 0) return 0;
+  const long nvf = len / 16;
+  return (nvf * v.fbits + 31) / 32 * 16 + ((len - nvf * 16) * v.fbits + 31) / 32;
+}
+
+// Work units of the flat compress: one unit per run of VLC_RUN vector rows of one lane, then one
+// per tail word
+constexpr int VLC_RUN = 8;
+__device__ __forceinline__ long compress_units(int len, const VlcParams &v)
+{
+  if (v.i2Len <= 0) return 0;
+  const long nvf = len / 16;
+  const long vrows = (nvf * v.fbits + 31) / 32;
+  return (vrows + VLC_RUN - 1) / VLC_RUN * 16 + ((len - nvf * 16) * v.fbits + 31) / 32;
+}
+
+// Rows r0 <= r < r1 of lane `lane`'s vector stream (word r*16 + lane), packed in one pass: the
+// fields overlapping the rows are encoded once each and shifted in MSB first; fields past the
+// lane's nvf are zero padding.  Same words as compress_word.
+__device__ __forceinline__ void compress_run(const double *__restrict__ in, int *__restrict__ o,
+                                             long nvf, const VlcParams &v, long r0, long r1,
+                                             int lane)
+{
+  const int fbits = v.fbits;
+  const long pos = r0 * 32;
+  long f = (pos <= (long)UINT32_MAX) ? (long)((uint32_t)pos / (uint32_t)fbits) : pos / fbits;
+  int nb = fbits - (int)(pos - f * fbits);  // bits of field f at or after pos
+  uint64_t acc = 0;
+  int nacc = 0;
+  long r = r0;
+  while (r < r1) {
+    const uint64_t val = f < nvf ? vlc_encode(in[f * 16 + lane], v, true) : 0;
+    while (nb > 0 && r < r1) {
+      const int take = min(nb, 32 - nacc);
+      acc = (acc << take) | ((val >> (nb - take)) & ((1ull << take) - 1));
+      nacc += take;
+      nb -= take;
+      if (nacc == 32) {
+        o[r * 16 + lane] = (int)(uint32_t)acc;
+        ++r;
+        acc = 0;
+        nacc = 0;
+      }
+    }
+    ++f;
+    nb = fbits;
+  }
+}
+
+// compress: one thread per output word, blockIdx.y = chunk.  nwords[c] = compress()'s return
+// value.
 __global__ void k_compress(int *__restrict__ out, long stride, const double *__restrict__ x,
                            int len, int nchunk, int e_max, const int *__restrict__ emi,
                            int *__restrict__ nwords, const int *__restrict__ e_max_dev)
@@ -335,30 +412,58 @@ __global__ void k_compress(int *__restrict__ out, long stride, const double *__r
   VlcParams v = vlc_params(e_max_dev ? *e_max_dev : e_max, emi[c]);
   const double *in = x + (long)c * len;
   int *o = out + (long)c * stride;
-  if (v.i2Len <= 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) nwords[c] = 0;
-    return;
-  }
-  const long nvf = len / 16;                 // fields per lane in the vector body
-  const long vrows = (nvf * v.fbits + 31) / 32;
-  const long ntail = len - nvf * 16;
-  const long twords = (ntail * v.fbits + 31) / 32;
-  const long total = vrows * 16 + twords;
+  const long total = compress_words(len, v);
   if (blockIdx.x == 0 && threadIdx.x == 0) nwords[c] = (int)total;
   for (long wi = blockIdx.x * (long)blockDim.x + threadIdx.x; wi < total;
-       wi += (long)gridDim.x * blockDim.x) {
-    uint32_t word;
-    if (wi < vrows * 16) {
-      long row = wi / 16;
-      int lane = (int)(wi % 16);
-      word = vlc_word(row * 32, v.fbits, nvf,
-                      [&](long f) { return vlc_encode(in[f * 16 + lane], v, true); });
-    } else {
-      long tw = wi - vrows * 16;
-      const double *tin = in + nvf * 16;
-      word = vlc_word(tw * 32, v.fbits, ntail, [&](long f) { return vlc_encode(tin[f], v, false); });
+       wi += (long)gridDim.x * blockDim.x)
+    o[wi] = (int)compress_word(in, len, v, wi);
+}
+
+// compress over all chunks as one range of work units (upre: exclusive prefix of the chunks'
+// unit counts, nchunk + 1 entries, from k_vlc_plan): every block has the same share of units, a
+// contiguous span, whatever the chunks' field widths.  A thread finds the chunk of its first unit
+// by bisection and then walks forward: chunks hold thousands of units, so the walk rarely moves.
+__global__ void k_compress_flat(int *__restrict__ out, long stride, const double *__restrict__ x,
+                                int len, int nchunk, const int *__restrict__ e_max_dev,
+                                const int *__restrict__ emi, const long *__restrict__ upre)
+{
+  const long total = upre[nchunk];
+  const long span = (total + gridDim.x - 1) / gridDim.x;
+  const long u1 = min(total, (blockIdx.x + 1) * span);
+  long u = blockIdx.x * span + threadIdx.x;
+  if (u >= u1) return;
+  int c = 0, hi = nchunk - 1;  // the last chunk c with upre[c] <= u: the one holding unit u
+  while (c < hi) {
+    const int mid = (c + hi + 1) >> 1;
+    if (upre[mid] <= u) c = mid;
+    else hi = mid - 1;
+  }
+  const int e_max = *e_max_dev;
+  const long nvf = len / 16;
+  long first = upre[c], next = upre[c + 1];
+  VlcParams v = vlc_params(e_max, emi[c]);
+  long vrows = (nvf * v.fbits + 31) / 32, nvu = (vrows + VLC_RUN - 1) / VLC_RUN * 16;
+  for (; u < u1; u += blockDim.x) {
+    if (next <= u) {
+      do {
+        ++c;
+        next = upre[c + 1];
+      } while (next <= u);
+      first = upre[c];
+      v = vlc_params(e_max, emi[c]);
+      vrows = (nvf * v.fbits + 31) / 32;
+      nvu = (vrows + VLC_RUN - 1) / VLC_RUN * 16;
     }
-    o[wi] = (int)word;
+    const long q = u - first;
+    const double *in = x + (long)c * len;
+    int *o = out + (long)c * stride;
+    if (q < nvu) {
+      const long r0 = (q >> 4) * VLC_RUN;
+      compress_run(in, o, nvf, v, r0, min(vrows, r0 + VLC_RUN), (int)(q & 15));
+    } else {
+      const long wi = vrows * 16 + (q - nvu);
+      o[wi] = (int)compress_word(in, len, v, wi);
+    }
   }
 }
 
@@ -396,6 +501,18 @@ __device__ __forceinline__ double vlc_decode(uint64_t fv, const VlcParams &v, bo
   return __dadd_rn(__dmul_rn((double)i1, v.d1), __dmul_rn((double)i2, v.d2));  // :529-531
 }
 
+// Element e of a chunk decoded from its stream w (compress.c:428-544)
+__device__ __forceinline__ double decompress_elem(const int *__restrict__ w, int len,
+                                                  const VlcParams &v, long e)
+{
+  if (v.i2Len <= 0) return 0.0;
+  const long nvf = len / 16;
+  if (e < nvf * 16)
+    return vlc_decode(vlc_field(w + e % 16, 16, e / 16, v.fbits), v, true);
+  const long vrows = (nvf * v.fbits + 31) / 32;
+  return vlc_decode(vlc_field(w + vrows * 16, 1, e - nvf * 16, v.fbits), v, false);
+}
+
 __global__ void k_decompress(double *__restrict__ out, const int *__restrict__ in, long stride,
                              int len, int nchunk, int e_max, const int *__restrict__ emi,
                              const int *__restrict__ e_max_dev)
@@ -405,24 +522,34 @@ __global__ void k_decompress(double *__restrict__ out, const int *__restrict__ i
   VlcParams v = vlc_params(e_max_dev ? *e_max_dev : e_max, emi[c]);
   double *o = out + (long)c * len;
   const int *w = in + (long)c * stride;
-  const long nvf = len / 16;
-  const long vrows = (nvf * v.fbits + 31) / 32;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < len;
-       e += (long)gridDim.x * blockDim.x) {
-    if (v.i2Len <= 0) {
-      o[e] = 0.0;
-      continue;
+       e += (long)gridDim.x * blockDim.x)
+    o[e] = decompress_elem(w, len, v, e);
+}
+
+// decompress over all chunks as one range of elements, a contiguous span per block
+__global__ void k_decompress_flat(double *__restrict__ out, const int *__restrict__ in,
+                                  long stride, int len, int nchunk,
+                                  const int *__restrict__ e_max_dev, const int *__restrict__ emi)
+{
+  const long total = (long)nchunk * len;
+  const long span = (total + gridDim.x - 1) / gridDim.x;
+  const long g1 = min(total, (blockIdx.x + 1) * span);
+  long g = blockIdx.x * span + threadIdx.x;
+  if (g >= g1) return;
+  const int e_max = *e_max_dev;
+  int c = (int)(g / len);
+  long e = g - (long)c * len;
+  VlcParams v = vlc_params(e_max, emi[c]);
+  for (; g < g1; g += blockDim.x, e += blockDim.x) {
+    if (e >= len) {
+      do {
+        e -= len;
+        ++c;
+      } while (e >= len);
+      v = vlc_params(e_max, emi[c]);
     }
-    double val;
-    if (e < nvf * 16) {
-      long f = e / 16;
-      int lane = (int)(e % 16);
-      val = vlc_decode(vlc_field(w + lane, 16, f, v.fbits), v, true);
-    } else {
-      long f = e - nvf * 16;
-      val = vlc_decode(vlc_field(w + vrows * 16, 1, f, v.fbits), v, false);
-    }
-    o[e] = val;
+    out[g] = decompress_elem(in + (long)c * stride, len, v, e);
   }
 }
 
@@ -636,7 +763,8 @@ extern "C" int soik_decompress(double *out, const int *in, long stride, int len,
 //      [1+2S .. 1+2S+P] segment boundaries; pcs: S + 1 doubles of scratch.
 __global__ void k_vlc_plan(const int *__restrict__ exps, int S, int P, int k, long l, long Mh,
                            double ratio, int *__restrict__ emi, int *__restrict__ out,
-                           double *__restrict__ pcs_g)
+                           double *__restrict__ pcs_g, long *__restrict__ upre,
+                           int *__restrict__ nwords)
 {
 #pragma clang fp contract(off)
   // segment maxima and prefix costs in LDS when S is small enough (the serial walk below then
@@ -658,6 +786,36 @@ __global__ void k_vlc_plan(const int *__restrict__ exps, int S, int P, int k, lo
   __syncthreads();
   const int total = smax;
   for (int c = threadIdx.x; c < S * P; c += blockDim.x) emi[c] = inl ? segs[c / P] : out[1 + c / P];
+  if (upre) {
+    // work units per chunk (all chunks of a segment alike) and their exclusive prefix over the
+    // chunks, for k_compress_flat; nwords = compress()'s return value per chunk
+    const int len = (int)(2 * l);
+    __shared__ long useg_total;
+    if (threadIdx.x == 0) {
+      long acc = 0;
+      for (int s = 0; s < S; ++s) {
+        if (inl) pcs_s[s] = (double)acc;  // reused below as each segment's first unit (exact)
+        acc += compress_units(len, vlc_params(total, inl ? segs[s] : out[1 + s])) * P;
+      }
+      useg_total = acc;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < S * P; c += blockDim.x) {
+      const int s = c / P, p = c % P;
+      const VlcParams v = vlc_params(total, inl ? segs[s] : out[1 + s]);
+      long first = 0;
+      if (inl) {
+        first = (long)pcs_s[s];
+      } else {
+        for (int q = 0; q < s; ++q)
+          first += compress_units(len, vlc_params(total, out[1 + q])) * P;
+      }
+      upre[c] = first + compress_units(len, v) * p;
+      nwords[c] = (int)compress_words(len, v);
+    }
+    if (threadIdx.x == 0) upre[S * P] = useg_total;
+    __syncthreads();
+  }
   if (threadIdx.x != 0) return;
   out[0] = total;
   pcs[0] = 0;
@@ -691,10 +849,29 @@ __global__ void k_vlc_plan(const int *__restrict__ exps, int S, int P, int k, lo
 }
 
 extern "C" int soik_vlc_plan(const int *exps, int S, int P, int k, long l, long Mh, double ratio,
-                             int *emi, int *out, double *pcs, hipStream_t st)
+                             int *emi, int *out, double *pcs, long *upre, int *nwords,
+                             hipStream_t st)
 {
   hipLaunchKernelGGL(k_vlc_plan, dim3(1), dim3(256), 0, st, exps, S, P, k, l, Mh, ratio, emi, out,
-                     pcs);
+                     pcs, upre, nwords);
+  return (int)hipGetLastError();
+}
+
+// Flat compress / decompress of the nchunk chunks of one VLC exchange (after soik_vlc_plan)
+extern "C" int soik_compress_flat(int *out, long stride, const double *x, int len, int nchunk,
+                                  const int *e_max_dev, const int *emi, const long *upre,
+                                  hipStream_t st)
+{
+  hipLaunchKernelGGL(k_compress_flat, dim3(2048), dim3(256), 0, st, out, stride, x, len, nchunk,
+                     e_max_dev, emi, upre);
+  return (int)hipGetLastError();
+}
+
+extern "C" int soik_decompress_flat(double *out, const int *in, long stride, int len, int nchunk,
+                                    const int *e_max_dev, const int *emi, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_decompress_flat, dim3(2048), dim3(256), 0, st, out, in, stride, len,
+                     nchunk, e_max_dev, emi);
   return (int)hipGetLastError();
 }
 

@@ -98,6 +98,66 @@ def test_batched_device_codec(soi, orc, torch_cuda):
     assert list(ex.cpu().numpy()) == [orc.max_exponent(x[c * length:(c + 1) * length]) for c in range(nch)]
 
 
+def test_flat_codec_matches_batched(soi, orc, torch_cuda):
+    """compute_soi's VLC exchange runs the planning kernel, then compress and decompress as
+    flat grids over all chunks (per-lane runs of words).  Their words, word counts and decoded
+    doubles must equal the per-chunk batched codec's bit for bit, chunk by chunk, with field
+    widths from 56 bits down to none (a segment 60 binades below the largest) and a tail of 8
+    elements per chunk."""
+    torch = torch_cuda
+    L = soi.lib()
+    c_long, c_int, c_void, c_dbl = ctypes.c_long, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+    L.soik_max_exponent.argtypes = [c_void, c_int, c_int, c_void, c_void]
+    L.soik_vlc_plan.argtypes = [c_void, c_int, c_int, c_int, c_long, c_long, c_dbl, c_void,
+                                c_void, c_void, c_void, c_void, c_void]
+    L.soik_compress_flat.argtypes = [c_void, c_long, c_void, c_int, c_int, c_void, c_void,
+                                     c_void, c_void]
+    L.soik_decompress_flat.argtypes = [c_void, c_void, c_long, c_int, c_int, c_void, c_void,
+                                       c_void]
+    S, P, l = 8, 4, 1004
+    k, length, nch, stride = S // P, 2 * l, S * P, 4 * l
+    rng = np.random.default_rng(11)
+    seg_e = [3, 2, 0, -7, -20, -31, -45, -60]
+    x = np.concatenate([_arr(rng, length, seg_e[c // P] - int(rng.integers(0, 3)))
+                        for c in range(nch)])
+    print(f"case flat codec S={S} P={P} l={l}", flush=True)
+    xt = to_dev(torch, x)
+    i32 = dict(dtype=torch.int32, device="cuda")
+    exps, emi = torch.zeros(nch, **i32), torch.zeros(nch, **i32)
+    plan = torch.zeros(1 + 2 * S + P + 1, **i32)
+    nw_flat, nw_ref = torch.zeros(nch, **i32), torch.zeros(nch, **i32)
+    pcs = torch.zeros(S + 1, dtype=torch.float64, device="cuda")
+    upre = torch.zeros(nch + 1, dtype=torch.int64, device="cuda")
+    w_flat = torch.zeros(nch * stride, **i32)
+    w_ref = torch.zeros(nch * stride, **i32)
+    assert L.soik_max_exponent(xt.data_ptr(), length, nch, exps.data_ptr(), None) == 0
+    assert L.soik_vlc_plan(exps.data_ptr(), S, P, k, l, P * l, 1.0, emi.data_ptr(),
+                           plan.data_ptr(), pcs.data_ptr(), upre.data_ptr(),
+                           nw_flat.data_ptr(), None) == 0
+    assert L.soik_compress_flat(w_flat.data_ptr(), stride, xt.data_ptr(), length, nch,
+                                plan.data_ptr(), emi.data_ptr(), upre.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    e_max = int(plan[0].item())
+    assert e_max == orc.max_exponent(x)
+    assert L.soi_compress_batch(w_ref.data_ptr(), stride, xt.data_ptr(), length, nch, e_max,
+                                emi.data_ptr(), nw_ref.data_ptr(), None) == 0
+    y_flat = torch.zeros(nch * length, dtype=torch.float64, device="cuda")
+    y_ref = torch.zeros(nch * length, dtype=torch.float64, device="cuda")
+    assert L.soik_decompress_flat(y_flat.data_ptr(), w_flat.data_ptr(), stride, length, nch,
+                                  plan.data_ptr(), emi.data_ptr(), None) == 0
+    assert L.soi_decompress_batch(y_ref.data_ptr(), w_ref.data_ptr(), stride, length, nch, e_max,
+                                  emi.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    nwf, nwr = nw_flat.cpu().numpy(), nw_ref.cpu().numpy()
+    assert np.array_equal(nwf, nwr)
+    assert nwr[0] > 0 and nwr[-1] == 0  # widest fields in segment 0, none in the last
+    wf = w_flat.cpu().numpy().reshape(nch, stride)
+    wr = w_ref.cpu().numpy().reshape(nch, stride)
+    for c in range(nch):
+        assert np.array_equal(wf[c, :nwr[c]], wr[c, :nwr[c]]), c
+    assert torch.equal(y_flat.view(torch.int64), y_ref.view(torch.int64))
+
+
 def _banded(N, S, seed, ntones=256):
     """Tones inside one band of width N/S plus white noise at -60 dB (the banded VLC workload)."""
     rng = np.random.default_rng(seed)

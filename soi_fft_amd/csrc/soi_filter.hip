@@ -21,6 +21,7 @@
 // loaded into registers while this tile computes, and barriers wait on LDS traffic only.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <type_traits>
 #include "soi_common.hpp"
 #include "soi_kernels.h"
@@ -874,6 +875,218 @@ __global__ __launch_bounds__(mfma32_threads(S, LPW), mfma32_threads(S, LPW) / 25
 }
 
 // ---------------------------------------------------------------------------------------------
+// The same complex64 filter software-pipelined across tiles: the epilogue of tile t (post phase,
+// LPW-point DFTs, exchange, NW-point DFT and stores) shares its two barrier intervals with the
+// MFMA chain of tile t + 1 (ten K steps in each).  Waves 0-3 and 4-7 (one of each on every SIMD)
+// take the two halves of an interval in opposite orders, so that one wave's VALU and LDS work
+// issues while the other wave's MFMAs occupy the matrix core.  The accumulators of tiles t and
+// t + 1 are both live (2 x 4 x LPW registers).
+//
+// Step t (tile t's accumulators in acc, tile t + 1's rows in buffer NB, the other buffer free):
+//   loads of tile t + 2 into the free buffer
+//   | MFMA ks 0-9 of tile t + 1   and   post + LPW-point DFTs + exchange writes of tile t |
+//   barrier (exchange rows of tile t published)
+//   | MFMA ks 10-19 of tile t + 1  and   exchange reads, NW-point DFTs, stores of tile t  |
+//   wait for the loads of tile t + 2, barrier (buffers and exchange rows free)
+// ---------------------------------------------------------------------------------------------
+template <int S, int LPW>
+__global__ __launch_bounds__(mfma32_threads(S, LPW), mfma32_threads(S, LPW) / 256) void k_filter_mfma32p(
+    const float2 *__restrict__ x, float2 *__restrict__ at, const float *__restrict__ wt,
+    const float2 *__restrict__ post, const float2 *__restrict__ twS, long M, long Mh, long nJ,
+    int tiles_per_block, SoiBatch b)
+{
+  typedef Mfma32Cfg<S, LPW> CF;
+  x = batch_ptr(x, b.x);
+  at = batch_ptr(at, b.at);
+  constexpr int NK = CF::NK, CH = CF::CH, CP = CF::CP, NW = CF::NWAVES, GP = CF::GP;
+  constexpr int KH = NK / 2;
+  static_assert(NK % 2 == 0, "two halves of the K steps");
+  gf32x2 *const atg = (gf32x2 *)at;
+  __shared__ __align__(16) float2 raw0[CF::RAWN];
+  __shared__ __align__(16) float2 raw1[CF::RAWN];
+  __shared__ __align__(16) float2 G[CF::GR * GP];
+  __shared__ __align__(16) float2 tw[S];
+  __shared__ __align__(16) float2 pfl[5 * S];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int n = lane & 15, g = lane >> 4;
+  // the wave group that runs its epilogue first (wave-uniform: a scalar branch)
+  const bool late = (__builtin_amdgcn_readfirstlane(wave) >> 2) & 1;
+
+  float wb[LPW][NK];  // B operands of this wave's lanes i = NW li + wave
+#pragma unroll
+  for (int li = 0; li < LPW; ++li)
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) wb[li][ks] = wt[((li * NW + wave) * NK + ks) * 64 + lane];
+  for (int e = tid; e < S; e += CF::T) tw[e] = twS[e];
+  for (int e = tid; e < 5 * S; e += CF::T) pfl[e] = post[e];
+
+  const long ntiles = (nJ + CF::TJ - 1) / CF::TJ;
+  const long t0 = (long)blockIdx.x * tiles_per_block;
+  const long t1 = min(ntiles, t0 + tiles_per_block);
+  const long N = M * S;
+  if (t0 >= t1) return;
+
+  auto dma = [&](float2 *dst, long g0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int c = wave; c < CF::NDMA; c += NW) {
+      long gi = g0 + c * CH + lane * CF::EPL;  // N and g0 are multiples of EPL
+      if (gi >= N) gi -= N;
+      __builtin_amdgcn_global_load_lds((const void *)(x + gi), (lds_void_t *)(dst + c * CP), 16, 0, 0);
+    }
+  };
+
+  const int am = n >> 1, apart = n & 1;
+  const int v0 = (12 * am + g) * S + wave;
+
+  // K steps [KS0, KS0 + KH) of a tile whose rows are in buffer BUF, accumulated into acc
+  auto mfma_half = [&](f32x4 (&acc)[LPW], auto buf_tag, auto half_tag) __attribute__((always_inline)) {
+    constexpr int BUF = decltype(buf_tag)::value;
+    constexpr int KS0 = decltype(half_tag)::value * KH;
+    const float *curf = reinterpret_cast<const float *>(BUF ? raw1 : raw0);
+    int vb = v0;
+    asm volatile("" : "+v"(vb));
+    constexpr bool FAST = (4 * S) % CH == 0;
+    const int abase = 2 * ((vb / CH) * CP + vb % CH) + apart;
+#pragma unroll
+    for (int ks = KS0; ks < KS0 + KH; ++ks) {
+#pragma unroll
+      for (int li = 0; li < LPW; ++li) {
+        float a;
+        if constexpr (FAST) {
+          a = curf[abase + 2 * (ks * (4 * S / CH) * CP + li * NW)];
+        } else {
+          const int e = vb + 4 * ks * S + li * NW;
+          a = curf[2 * ((e / CH) * CP + e % CH) + apart];
+        }
+        acc[li] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wb[li][ks], acc[li], 0, 0, 0);
+      }
+    }
+  };
+
+  // post phase and LPW-point DFTs of a tile's accumulators into the exchange rows
+  auto epi_write = [&](const f32x4 (&acc)[LPW]) __attribute__((always_inline)) {
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+    const int ln = tv & 63, nv = ln & 15, gv = ln >> 4;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float2 v[LPW];
+#pragma unroll
+      for (int li = 0; li < LPW; ++li)
+        v[li] = cmul(float2{acc[li][2 * h], acc[li][2 * h + 1]}, pfl[(nv % 5) * S + li * NW + wave]);
+      dft_r<LPW>(v);
+      if (nv < 15) {
+        float2 *gr = G + (15 * (2 * gv + h) + nv) * GP + wave;
+        gr[0] = v[0];
+#pragma unroll
+        for (int s1 = 1; s1 < LPW; ++s1) gr[s1 * NW] = cmul(v[s1], tw[(wave * s1) % S]);
+      }
+    }
+  };
+
+  // NW-point DFTs over the waves for each (R, s1) of a tile, stored segment-major
+  auto epi_store = [&](long tile, auto part_tag) __attribute__((always_inline)) {
+    constexpr bool PART = decltype(part_tag)::value;
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+    const long R0 = tile * CF::TJ * 5;
+#pragma unroll
+    for (int c = 0; c < CF::NC; ++c) {
+      int e = tv + c * CF::T;
+      if constexpr (CF::DUP) {
+        if (e >= CF::NTASK) e -= CF::DUPD;
+      }
+      if (CF::DUP || CF::NTASK % CF::T == 0 || e < CF::NTASK) {
+        const int r = e % CF::GR, s1 = e / CF::GR;
+        float2 u[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) u[w] = G[r * GP + s1 * NW + w];
+        dft_r<NW>(u);
+        if (!PART || tile * CF::TJ + r / 5 < nJ) {
+#pragma unroll
+          for (int s2 = 0; s2 < NW; ++s2)
+            store_nt_global(atg + (long)(s1 + LPW * s2) * Mh + R0 + r, u[s2]);
+        }
+      }
+    }
+  };
+
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+
+  // prologue: the first tile's rows, its MFMA chain, the second tile's rows
+  dma(raw0, t0 * CF::TJ * 4 * S);
+  wait_vmcnt<0>();
+  lds_barrier();
+  if (t0 + 1 < t1) dma(raw1, (t0 + 1) * CF::TJ * 4 * S);
+  f32x4 acc[LPW];
+#pragma unroll
+  for (int li = 0; li < LPW; ++li) acc[li] = f32x4{0, 0, 0, 0};
+  mfma_half(acc, I0{}, I0{});
+  mfma_half(acc, I0{}, I1{});
+  wait_vmcnt<0>();
+  lds_barrier();  // the second tile's rows are in; raw0 is free
+
+  // step t: whole tile t with a tile t + 1 behind it; NB = buffer of tile t + 1.  LATE: this
+  // wave runs the epilogue before the MFMA half in the first interval and after it in the second.
+  auto step = [&](long tile, auto nb_tag, auto late_tag) __attribute__((always_inline)) {
+    constexpr int NB = decltype(nb_tag)::value;
+    constexpr bool LATE = decltype(late_tag)::value;
+    using BN = std::integral_constant<int, NB>;
+    if (tile + 2 < t1) dma(NB ? raw0 : raw1, (tile + 2) * CF::TJ * 4 * S);
+    f32x4 accn[LPW];
+#pragma unroll
+    for (int li = 0; li < LPW; ++li) accn[li] = f32x4{0, 0, 0, 0};
+    if constexpr (LATE) {
+      epi_write(acc);
+      mfma_half(accn, BN{}, I0{});
+    } else {
+      mfma_half(accn, BN{}, I0{});
+      epi_write(acc);
+    }
+    lds_barrier();  // the exchange rows of tile t are complete
+    if constexpr (LATE) {
+      mfma_half(accn, BN{}, I1{});
+      epi_store(tile, std::false_type{});
+    } else {
+      epi_store(tile, std::false_type{});
+      mfma_half(accn, BN{}, I1{});
+    }
+    // the loads of tile t + 2 are complete once at most this wave's stores of tile t are
+    // outstanding (issued after them); unconditional, so every path back to the loop head waits
+    asm volatile("" ::: "memory");
+    if constexpr (CF::DUP) wait_vmcnt<CF::NC * NW>();
+    else wait_vmcnt<0>();
+    lds_barrier();  // tile t + 2's rows are in; the exchange rows and buffer NB are free
+#pragma unroll
+    for (int li = 0; li < LPW; ++li) acc[li] = accn[li];
+  };
+
+  // whole tiles with a successor, two per iteration (static buffer names); then the last tile.
+  // One copy of the loop per wave group: a branch inside the loop between the two orders would
+  // make the compiler lose track of which buffer a pending load writes (and wait for all of them).
+  const long tfull = min(t1, nJ / CF::TJ);
+  auto run = [&](auto late_tag) __attribute__((always_inline)) {
+    long tile = t0;
+    for (; tile + 2 < t1 && tile + 1 < tfull; tile += 2) {
+      step(tile, I1{}, late_tag);
+      step(tile + 1, I0{}, late_tag);
+    }
+    if (tile + 1 < t1 && tile < tfull) {
+      step(tile, I1{}, late_tag);
+      ++tile;
+    }
+    // tile == t1 - 1: its accumulators are in acc
+    epi_write(acc);
+    lds_barrier();
+    if (tile < tfull) epi_store(tile, std::false_type{});
+    else epi_store(tile, std::true_type{});
+  };
+  if (late) run(std::true_type{});
+  else run(std::false_type{});
+}
+
+// ---------------------------------------------------------------------------------------------
 // Complex128 real-tap filter on MFMA (v_mfma_f64_16x16x4_f64, FP64 products and accumulation).
 // The GEMM of each lane i is the one of the FP32 MFMA form with the rows ordered part-major,
 //   C[8 part + m][theta + 5D] = sum_k x_i[4 J0 + 12 m + k].part * t_i[theta][k - 4D],
@@ -1197,7 +1410,12 @@ template <int S, int LPW> static int launch_mfma32(const SoiFilterArgs *a, hipSt
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
-  hipLaunchKernelGGL((k_filter_mfma32<S, LPW>), dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1),
+  static const int pipe = [] {
+    const char *e = getenv("SOI_FILTER_MFMA32_PIPE");
+    return e ? atoi(e) : 1;
+  }();
+  auto kern = pipe ? k_filter_mfma32p<S, LPW> : k_filter_mfma32<S, LPW>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb, a->b.n > 0 ? a->b.n : 1),
                      dim3(CF::T), 0, st, (const float2 *)a->x, (float2 *)a->at,
                      (const float *)a->wt, (const float2 *)a->post, (const float2 *)a->twS, a->M,
                      a->Mh, a->nJ, tpb, a->b);

@@ -221,7 +221,7 @@ static int pick_tile(int n, long other, int cap)
  * variant, row tile 8 or 4.  The planned choice is entry 0 of the table. */
 static void fft_variants(soi_plan_t *pl)
 {
-  int cols[5][2], nc = 0, rows[2], nr = 0;
+  int cols[6][2], nc = 0, rows[2], nr = 0;
   const int n1 = pl->n1, n2 = pl->n2;
   const int c4 = n1 == 1024 || n1 == 1280 || n1 == 2048 || n1 == 2560;
   cols[nc][0] = pl->Tc, cols[nc][1] = pl->clay, ++nc;
@@ -236,8 +236,10 @@ static void fft_variants(soi_plan_t *pl)
   if ((n1 == 2048 || n1 == 2560) && n2 % 8 == 0 && pl->clay != 1)
     cols[nc][0] = 8, cols[nc][1] = 1, ++nc;
   if (n1 == 2560 && n2 % 4 == 0 && pl->clay != 2 && nc < 5) cols[nc][0] = 4, cols[nc][1] = 2, ++nc;
-  if (n1 == 2560 && pl->f32 && n2 % 8 == 0 && pl->clay != 3 && nc < 5)
+  if (n1 == 2560 && pl->f32 && n2 % 8 == 0 && pl->clay != 3 && nc < 6)
     cols[nc][0] = 8, cols[nc][1] = 3, ++nc;
+  if (n1 == 2560 && !pl->f32 && n2 % 4 == 0 && pl->clay != 4 && nc < 6)
+    cols[nc][0] = 4, cols[nc][1] = 4, ++nc;
   rows[nr++] = pl->Tr;
   if (pl->Tr != 8 && n2 != 2560 && n1 % 8 == 0) rows[nr++] = 8;
   if (pl->Tr != 4 && (n2 == 1024 || n2 == 1280 || n2 == 2048 || n2 == 2560) && n1 % 4 == 0)
@@ -307,13 +309,15 @@ static int plan_fft(soi_plan_t *pl)
         pl->Tr = pl->n1 % 8 ? 10 : 8;
       }
       /* column-pass kernel variant: 1 = one padded LDS run per sequence (8-wide tiles of 2048
-       * or 2560); 2 = one 4-wide block per CU with the whole register budget (2560, the
-       * complex128 default there: 20 values per thread do not fit 1024-thread blocks).
+       * or 2560); 2 = one 4-wide block per CU with the whole register budget (2560: 20
+       * values per thread do not fit 1024-thread blocks); 4 = the same exchanging whole
+       * complex128 values in all 160 KiB of LDS (2560, the complex128 default: one round per
+       * exchange instead of a real and an imaginary one, 1.59 -> 1.51 ms at N = 2^27).
        * Complex64 2560-point columns default to 4-wide tiles in two exchange groups, two
        * blocks per CU (their whole-complex exchanges fit in 44 KB).  SOI_FFT_CVAR overrides
        * for tuning */
       const char *el = getenv("SOI_FFT_CVAR");
-      int cv = el && !stile ? atoi(el) : (pl->n1 == 2560 && !pl->f32 ? 2 : 0);
+      int cv = el && !stile ? atoi(el) : (pl->n1 == 2560 && !pl->f32 ? 4 : 0);
       if (!el && !ec && pl->n1 == 2560 && pl->f32 && pl->n2 % 4 == 0) pl->Tc = 4;
       pl->clay = 0;
       if (cv == 1 && pl->Tc == 8 && (pl->n1 == 2048 || pl->n1 == 2560)) pl->clay = 1;
@@ -324,6 +328,10 @@ static int plan_fft(soi_plan_t *pl)
       if (cv == 3 && pl->n1 == 2560 && pl->f32) {
         pl->clay = 3;
         pl->Tc = 8;
+      }
+      if (cv == 4 && pl->n1 == 2560 && !pl->f32) {
+        pl->clay = 4;
+        pl->Tc = 4;
       }
       int lb = 0;
       while ((1L << lb) * (1L << lb) < Mh) ++lb;

@@ -125,6 +125,19 @@ template <int L, int TG> struct QFast8 {
   static constexpr int size = (L + L / 8) * TG;
   static __device__ __forceinline__ int at(int pos, int q) { return (pos + (pos >> 3)) * TG + q; }
 };
+//   QFull:    whole complex values, no padding: L*TG*16 bytes (160 KiB for 2560 points x 4
+//             complex128 sequences, the most one block may hold).  Position bits 0-3 are XORed
+//             with bits 4-7 (a bijection within each run of 16) to spread the radix-10 stage's
+//             writes, 10 positions apart, over the banks.
+template <int L, int TG> struct QFull {
+  static_assert(L % 16 == 0, "swizzle within runs of 16");
+  static constexpr bool full = true;
+  static constexpr int size = L * TG;
+  static __device__ __forceinline__ int at(int pos, int q)
+  {
+    return ((pos & ~15) | ((pos ^ (pos >> 4)) & 15)) * TG + q;
+  }
+};
 template <int L, int TG> struct SeqMajor {
   static constexpr bool full = false;
   static constexpr int LP0 = L + L / 16;
@@ -134,10 +147,10 @@ template <int L, int TG> struct SeqMajor {
 };
 
 // Complex64 exchanges move whole complex values in one round when the layout fits in LDS: half
-// the LDS instructions and barriers of the real / imaginary rounds.  (Complex128 tiles keep the
-// rounds: exchanging them whole in two groups of the T sequences, the only way they fit,
-// measured slower for the 2560-point columns: 1.83 against 1.60 ms at N = 2^27.)  A layout may
-// set `full` to ask for whole exchanges of complex128 values as well.
+// the LDS instructions and barriers of the real / imaginary rounds.  Complex128 tiles keep the
+// rounds in the padded layouts (a padded 2560 x 4 tile exchanged whole needs two groups of
+// sequences, measured slower: 1.83 against 1.60 ms at N = 2^27); the unpadded QFull layout sets
+// `full` and holds a whole 2560 x 4 complex128 tile in 160 KiB (1.59 -> 1.51 ms).
 template <typename LY, typename C> constexpr bool exch_full()
 {
   return (sizeof(real_t<C>) == 4 || LY::full) && (size_t)LY::size * sizeof(C) <= 160 * 1024;
@@ -585,7 +598,8 @@ extern "C" int soik_fft_cols_short_tile(int L)
 // Tile width T: 8 (128-byte runs) or 4 (64-byte runs).  variant selects a tuning alternative
 // for some lengths: 1 = one padded LDS run per sequence (2048, 2560 at T = 8); 2 = the whole
 // 256-VGPR budget for one 4-wide block per CU (2560 at T = 4); 3 = radices 4.4.4.4.10 (2560 at
-// T = 8, complex64 only).
+// T = 8, complex64 only); 4 = one 4-wide block per CU exchanging whole complex128 values
+// (QFull, 2560 at T = 4).
 template <typename C>
 static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, int n2, int lb1,
                     long Mh, int S, int T, int variant, C *tout, SoiBatch b, hipStream_t st)
@@ -602,6 +616,9 @@ static int cols_reg(C *data, const C *tw1, const C *thi, const C *tlo, int n1, i
   COLS(0, 2048, 16, 4, 1, 2, QFast, 16, 16, 8) COLS(0, 2560, 20, 4, 2, 2, QFast, 10, 4, 4, 4, 4)
   COLS(1, 2048, 16, 8, 1, 1, SeqMajor, 16, 16, 8) COLS(1, 2560, 20, 8, 2, 1, SeqMajor, 10, 4, 4, 4, 4)
   COLS(2, 2560, 20, 4, 1, 1, QFast, 10, 4, 4, 4, 4)
+  if constexpr (sizeof(C) == 16) {  // complex128: whole-value exchanges in the full 160 KiB
+    COLS(4, 2560, 20, 4, 1, 1, QFull, 10, 4, 4, 4, 4)
+  }
   if constexpr (sizeof(C) == 8) {  // complex64: the radix-10 stage last needs fewer registers
     COLS(3, 2560, 20, 8, 2, 1, QFast, 4, 4, 4, 4, 10)
   }

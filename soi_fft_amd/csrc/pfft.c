@@ -1164,6 +1164,7 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
   clear_err();
   soi_batch_t *b = NULL;
   int capturing = 0;
+  unsigned char *tsave = NULL; /* the descriptors' timing flags, restored on every exit */
   if (!descs || !bufs || count < 1) {
     set_err(SOI_ERR_ARGS, "soi_batch_create needs count >= 1 descriptors and buffers");
     return NULL;
@@ -1234,7 +1235,15 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
     HIPCHK(hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming));
   }
   /* timing events would become graph nodes: capture with timing off */
-  for (int i = 0; i < count; ++i) ((soi_plan_t *)descs[i]->plan)->timing = 0;
+  tsave = (unsigned char *)malloc((size_t)count);
+  if (!tsave) {
+    set_err(SOI_ERR_ALLOC, "host allocation failed");
+    goto fail_quiet;
+  }
+  for (int i = 0; i < count; ++i) {
+    tsave[i] = (unsigned char)((soi_plan_t *)descs[i]->plan)->timing;
+    ((soi_plan_t *)descs[i]->plan)->timing = 0;
+  }
   /* groups of transforms that run as one launch per kernel (blockIdx.y = member): member
    * pointer tables [x][alpha_tilde][tile buffer] in device memory */
   int *grp = (int *)malloc(sizeof(int) * count), *gsize = (int *)calloc(count, sizeof(int));
@@ -1281,10 +1290,20 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
     }
   }
   free(htab);
-  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-  capturing = 1;
-  HIPCHK(hipEventRecord(b->fork, b->stream));
-  for (int r = 0; r < b->nbranch; ++r) HIPCHK(hipStreamWaitEvent(b->branch[r], b->fork, 0));
+  {
+    hipError_t ce = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
+    if (ce == hipSuccess) {
+      capturing = 1;
+      ce = hipEventRecord(b->fork, b->stream);
+    }
+    for (int r = 0; r < b->nbranch && ce == hipSuccess; ++r)
+      ce = hipStreamWaitEvent(b->branch[r], b->fork, 0);
+    if (ce != hipSuccess) {
+      free(grp), free(gsize), free(gfirst), free(gwork), free(load);
+      set_err(SOI_ERR_HIP, "batch capture: %s", hipGetErrorString(ce));
+      goto fail_quiet;
+    }
+  }
   /* groups in decreasing work, each on the least loaded branch */
   for (int k = 0; k < b->ngroups; ++k) {
     int g = -1;
@@ -1314,6 +1333,8 @@ soi_batch_t *soi_batch_create(soi_desc_t **descs, cfft_complex_t **bufs, int cou
   capturing = 0;
   HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
   HIPCHK(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+  for (int i = 0; i < count; ++i) ((soi_plan_t *)descs[i]->plan)->timing = tsave[i];
+  free(tsave);
   return b;
 fail:
 fail_quiet:
@@ -1323,6 +1344,9 @@ fail_quiet:
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
   }
+  if (tsave)
+    for (int i = 0; i < count; ++i) ((soi_plan_t *)descs[i]->plan)->timing = tsave[i];
+  free(tsave);
   soi_batch_destroy(b);
   return NULL;
 }

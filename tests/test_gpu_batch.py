@@ -151,3 +151,27 @@ def test_batch_512_member_workload(soi, orc, torch_cuda, prec):
             batch.close()
         for d in descs:
             soi.free_soi_descriptor(d)
+
+
+def test_batch_create_keeps_timing(soi, torch_cuda):
+    """Capture runs with per-stage timing off (its events would become graph nodes); the
+    descriptors' own timing setting is restored afterwards."""
+    torch = torch_cuda
+    N, P = 2 ** 18, 8
+    d = soi.make_descriptor(N, P)
+    batch = None
+    try:
+        x = torch.empty(N, dtype=torch.complex128, device="cuda")
+        assert soi.lib().soi_fill_uniform(x.data_ptr(), N, 5, None) == 0
+        torch.cuda.synchronize()
+        soi.enable_timing(d, True)
+        batch = soi.Batch([d], [x.data_ptr()], autotune=False)
+        batch.run()
+        torch.cuda.synchronize()
+        soi.compute_soi(d, x.data_ptr())
+        t = soi.stage_times(d)
+        assert len(t) == 4 and t[0] > 0 and t[2] + t[3] > 0, t
+    finally:
+        if batch is not None:
+            batch.close()
+        soi.free_soi_descriptor(d)

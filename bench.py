@@ -8,11 +8,12 @@ the input: the work of every step is identical.  Per-kernel durations come from 
 the library records on its stream around each kernel during the timed steps.  Beside the
 headline FP64 form the line carries the FP32 precision form at the same shape, the batching form
 (512 transforms of uneven N and P, both precisions), the banded-spectrum VLC workload
-(N = 7*2^23, P = 64, mu = 8/7) and the CPU baseline (a C/OpenMP port of the reference's timed
-region on this host's cores).  Prints one JSON line.
+(N = 7*2^23, P = 64, mu = 8/7), rocFFT on the same input in both precisions (the Cooley-Tukey
+comparison) and the CPU baseline (a C/OpenMP port of the reference's timed region on this
+host's cores).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu-baseline] [--no-fp32]
-                    [--no-batch] [--no-vlc]
+                    [--no-batch] [--no-vlc] [--no-rocfft]
     python bench.py --copy-calibration ...   also runs one plain copy kernel (torch.neg over the
                                              N-vector: 16N bytes read, 16N written) before the
                                              timed steps, for calibrating HBM byte counters
@@ -47,6 +48,7 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-vlc", action="store_true")
+    ap.add_argument("--no-rocfft", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
     return ap.parse_args()
 
@@ -337,6 +339,38 @@ def bench_vlc(args, soi, torch):
             "note": "N=2^26 has no valid (P*d_mu) | M split at mu=8/7; 7*2^23 is the nearest"}
 
 
+def bench_rocfft(args, torch, x, soi_ms):
+    """The Cooley-Tukey comparison (the role of the reference test.c's MKL / MKL-CDFT leg,
+    test.c:246-271): rocFFT through torch.fft.fft (hipFFT) on the same N-point input, out of
+    place into a preallocated output, complex128 and complex64, timed with events like the SOI
+    steps.  soi_speedup = rocFFT ms / SOI ms of the same precision."""
+    out = {}
+    N = x.numel()
+    flop = 5.0 * N * math.log2(N)
+    for name, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        if name not in soi_ms:
+            continue
+        xi = x if dt == torch.complex128 else x.to(dt)
+        y = torch.empty_like(xi)
+        for _ in range(max(2, args.warmup)):
+            torch.fft.fft(xi, out=y)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            torch.fft.fft(xi, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        out[name] = {"ms_per_step": round(ms, 4), "GFLOP/s": round(flop / (ms * 1e-3) / 1e9, 2),
+                     "soi_speedup": round(ms / soi_ms[name], 3)}
+        del y, xi
+        torch.cuda.empty_cache()
+    out["note"] = "torch.fft.fft (hipFFT / rocFFT) of the same input, out of place"
+    return out
+
+
 def cpu_baseline(log2n, P):
     """The CPU performance port of the reference's timed region (test.c:389-404: compute_soi on
     a resident input, tables built beforehand; oracle/soi_cpu_port.c, C + OpenMP + AVX2) on this
@@ -437,6 +471,11 @@ def main():
     }
     if not args.no_fp32:
         out["fp32_split"] = bench_fp32(args, soi, torch, x, None, N, P, gflops)
+    if not args.no_rocfft:
+        soi_ms = {"fp64": ms}
+        if "fp32_split" in out:
+            soi_ms["fp32"] = out["fp32_split"]["ms_per_step"]
+        out["rocfft"] = bench_rocfft(args, torch, x, soi_ms)
     del x
     torch.cuda.empty_cache()
     if not args.no_batch:

@@ -41,12 +41,30 @@ typedef struct _complex_struct_t {          /* reference soi.h:222-225 */
   double im;
 } complex_struct_t;
 
-/* reference soi.h:227: C99 double complex.  C++ callers see a layout-identical struct. */
+/* reference intrinsic.h:4-5: PRECISION 2 (the default) is double precision, PRECISION 1 single.
+ * reference soi.h:227: cfft_complex_t is C99 double complex (C++ callers see a layout-identical
+ * struct).  A caller built with -DPRECISION=1 gets float complex buffers, and
+ * set_default_soi_descriptor selects the FP32 form for it (below). */
+#ifndef PRECISION
+#define PRECISION 2
+#endif
+#if PRECISION == 1
+#ifdef __cplusplus
+typedef struct _complex_struct32_t {
+  float re;
+  float im;
+} cfft_complex_t;
+#else
+#include <complex.h>
+typedef float complex cfft_complex_t;
+#endif
+#else
 #ifdef __cplusplus
 typedef complex_struct_t cfft_complex_t;
 #else
 #include <complex.h>
 typedef double complex cfft_complex_t;
+#endif
 #endif
 
 /* Precision / form selector (not in the reference, which is FP64 only).
@@ -116,6 +134,14 @@ typedef struct {
 /* pfft.c:84-138: n_mu=5, d_mu=4, tau=928/1024, sigma=373.7314, B=72, use_vlc=0,
  * comm_to_comp_cost_ratio=1; nulls the reusable buffers.  Also P=1, rank=0, k=1, N=0. */
 void set_default_soi_descriptor(soi_desc_t *desc);
+
+/* set_default_soi_descriptor with the precision field set as well (no counterpart in the
+ * reference).  A caller built with -DPRECISION=1 reaches it through set_default_soi_descriptor:
+ * the reference's single-precision build (intrinsic.h:34-56) maps onto the FP32 form. */
+void soi_set_default_descriptor_precision(soi_desc_t *desc, int precision);
+#if PRECISION == 1
+#define set_default_soi_descriptor(desc) soi_set_default_descriptor_precision((desc), SOI_PREC_FP32)
+#endif
 
 /* pfft.c:140-232: sets comm and k, segment boundaries p*k, allocates device buffers (unless
  * preallocated), builds windows and taps on the GPU in FP64, plans the FFTs.  The caller must

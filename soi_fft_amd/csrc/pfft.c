@@ -167,6 +167,12 @@ void set_default_soi_descriptor(soi_desc_t *desc)
   desc->window_mode = SOI_WINDOW_AUTO;
 }
 
+void soi_set_default_descriptor_precision(soi_desc_t *desc, int precision)
+{
+  set_default_soi_descriptor(desc);
+  desc->precision = precision;
+}
+
 /* Filter launch configuration (see soi_filter.hip).  Window evaluation: exact windows with real
  * taps for (5,4) and B >= 60, where they agree with the reference's window values to ~1e-14 in
  * the output; otherwise the reference's own window values as complex taps. */

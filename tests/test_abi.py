@@ -104,6 +104,36 @@ def test_reference_style_c_caller_links(tmp_path):
     assert exe.exists()
 
 
+PREC1_C = r"""
+#include <stdio.h>
+#include "soi.h"
+int main(void) {
+  soi_desc_t d;
+  set_default_soi_descriptor(&d);
+  printf("%d %zu %d\n", PRECISION, sizeof(cfft_complex_t), d.precision);
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("prec,size,form", [(None, 16, 0), (2, 16, 0), (1, 8, 1)])
+def test_precision_macro_maps_onto_the_forms(tmp_path, prec, size, form):
+    """reference intrinsic.h:4-5: a caller built with -DPRECISION=1 gets float complex buffers
+    and set_default_soi_descriptor selects the FP32 form; PRECISION 2 (or none) stays the FP64
+    drop-in.  The caller runs here: set_default_soi_descriptor makes no GPU call."""
+    src = tmp_path / "prec.c"
+    src.write_text(PREC1_C)
+    exe = tmp_path / "prec"
+    libdir = os.path.dirname(soi.LIB_PATH)
+    flags = [] if prec is None else [f"-DPRECISION={prec}"]
+    subprocess.check_call(["gcc", "-std=gnu11", *flags, f"-I{INC}", str(src), "-o", str(exe),
+                           f"-L{libdir}", "-lsoi_hip", f"-Wl,-rpath,{libdir}", "-lm"])
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    p, sz, fm = (int(t) for t in out.stdout.split())
+    assert (p, sz, fm) == (prec or 2, size, form)
+
+
 def _desc(**kw):
     d = soi.SoiDesc()
     soi.set_default_soi_descriptor(d)

@@ -88,6 +88,39 @@ def caller(tmp_path_factory, soi):
     return exe
 
 
+@pytest.fixture(scope="module")
+def caller32(tmp_path_factory, soi):
+    """The same caller built with -DPRECISION=1 (reference intrinsic.h:4-5): float complex
+    buffers, and set_default_soi_descriptor selects the FP32 form."""
+    d = tmp_path_factory.mktemp("caller32")
+    src = d / "caller.c"
+    src.write_text(CALLER_C)
+    exe = d / "caller32"
+    libdir = os.path.dirname(soi.LIB_PATH)
+    subprocess.check_call(["gcc", "-std=gnu11", "-DPRECISION=1", "-D__HIP_PLATFORM_AMD__",
+                           f"-I{INC}", f"-I{ROCM}/include", str(src), "-o", str(exe),
+                           f"-L{libdir}", "-lsoi_hip", f"-Wl,-rpath,{libdir}", f"-L{ROCM}/lib",
+                           "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lm"])
+    return exe
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_precision1_c_caller_runs_the_fp32_form(caller32, orc, tmp_path, where):
+    N, P, k = 2 ** 16, 8, 1
+    print(f"case C caller PRECISION=1 {where} N={N} P={P}", flush=True)
+    x = orc.uniform_input(N, 78).astype(np.complex64)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(fin)
+    r = subprocess.run([str(caller32), where, str(N), str(P), str(k), str(fin), str(fout)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    got = np.fromfile(fout, dtype=np.complex64).astype(np.complex128)
+    ref, _ = orc.compute_soi(orc.params(N, P, k), x.astype(np.complex128))
+    err = rel(got, ref)
+    print(f"C caller PRECISION=1 {where}: rel-L2 vs FP64 oracle {err:.3e}")
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("where", ["host", "device"])
 @pytest.mark.parametrize("N,P,k", [(2 ** 16, 8, 1), (2 ** 18, 4, 2)])
 def test_reference_style_c_caller_runs(caller, orc, tmp_path, where, N, P, k):

@@ -210,6 +210,37 @@ def bench_fp32(args, soi, torch, x64, ref, N, P, fp64_gflops):
                     "generation; W_inv applied in FP64"}
 
 
+def bench_fp32_bf16x3(args, soi, torch, x64, N, P):
+    """The FP32 form with the split-bf16 MFMA filter (SOI_FILTER_MFMA=1: every operand split
+    into hi + mid + lo bf16 pieces, six piece products on v_mfma_f32_16x16x32_bf16), opt-in, at
+    the same shape: its filter time and errors beside the default exact-FP32 MFMA filter."""
+    old = os.environ.get("SOI_FILTER_MFMA")
+    os.environ["SOI_FILTER_MFMA"] = "1"
+    try:
+        d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    finally:
+        if old is None:
+            del os.environ["SOI_FILTER_MFMA"]
+        else:
+            os.environ["SOI_FILTER_MFMA"] = old
+    form = soi.filter_form(d)
+    x = x64.to(torch.complex64)
+    ref32 = torch.fft.fft(x.to(torch.complex128))
+    y = x.clone()
+    torch.cuda.synchronize()
+    soi.compute_soi(d, y.data_ptr())
+    err = errors(torch, y.to(torch.complex128), ref32)
+    del ref32, y
+    torch.cuda.empty_cache()
+    a = argparse.Namespace(steps=max(3, args.steps // 2), warmup=min(args.warmup, 3))
+    bufs = step_buffers(torch, x, a.steps)
+    ms, acc, _ = timed_steps(a, soi, torch, d, bufs, x)
+    del bufs
+    soi.free_soi_descriptor(d)
+    return {"filter_form": form, "ms_per_step": round(ms, 4), "steps": a.steps,
+            "filter_sdft_ms": round(acc[0], 4), "errors_vs_rocfft_fp64": err}
+
+
 def bench_batch(args, soi, torch, prec):
     """The batching form: args.batch_count independent transforms, N uniform over {2^16, 3*2^15,
     2^18, 5*2^16, 2^20}, P uniform over {8, 16, 32} (rejecting (2^16, 32), whose M/P < B),
@@ -471,6 +502,7 @@ def main():
     }
     if not args.no_fp32:
         out["fp32_split"] = bench_fp32(args, soi, torch, x, None, N, P, gflops)
+        out["fp32_split"]["bf16x3_filter"] = bench_fp32_bf16x3(args, soi, torch, x, N, P)
     if not args.no_rocfft:
         soi_ms = {"fp64": ms}
         if "fp32_split" in out:

@@ -42,6 +42,14 @@ def rel(a, b):
     return r
 
 
+def rel_t(torch, a, b):
+    """rel() for device tensors, computed on the device (full-size outputs) and recorded too."""
+    r = float((torch.linalg.norm(a - b) / torch.linalg.norm(b)).item())
+    tid = os.environ.get("PYTEST_CURRENT_TEST", "?").rsplit(" (", 1)[0]
+    _RECORD.setdefault(tid, []).append(r)
+    return r
+
+
 def recorded():
     """{test id: [relative L2 errors of its comparisons, in order]}"""
     return _RECORD

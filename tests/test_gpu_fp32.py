@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_util import rel
+from gpu_util import rel, rel_t
 
 pytestmark = pytest.mark.gpu
 
@@ -143,3 +143,31 @@ def test_fp32_mfma_filter_matches_fp64_oracle(soi, orc, torch_cuda, monkeypatch,
     err = rel(got.astype(np.complex128), ref)
     print(f"  mfma fp32 rel-L2 vs fp64 oracle {err:.3e}", flush=True)
     assert err < FP32_TOL
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_fp32_mfma_filter_forms_full_size(soi, torch_cuda, monkeypatch, mode):
+    """Both MFMA forms of the complex64 filter at N = 2^27, P = 32 (the FP32-split workload):
+    SOI_FILTER_MFMA=1 (operands split into bf16 pieces) and =2 (exact FP32 products, the
+    default), against rocFFT's FP64 FFT of the same complex64 input at the FP32 bound."""
+    torch = torch_cuda
+    monkeypatch.setenv("SOI_FILTER_MFMA", mode)
+    N, P = 2 ** 27, 32
+    print(f"case N=2^27 P=32 mfma filter mode {mode}", flush=True)
+    x = torch.empty(N, dtype=torch.complex128, device="cuda")
+    assert soi.lib().soi_fill_uniform(x.data_ptr(), N, 404, None) == 0
+    x32 = x.to(torch.complex64)
+    del x
+    ref = torch.fft.fft(x32.to(torch.complex128))
+    d = soi.make_descriptor(N, P, precision=soi.SOI_PREC_FP32)
+    try:
+        assert soi.filter_form(d) == int(mode)
+        y = x32.clone()
+        torch.cuda.synchronize()
+        soi.compute_soi(d, y.data_ptr())
+        torch.cuda.synchronize()
+        err = rel_t(torch, y.to(torch.complex128), ref)
+        print(f"  N=2^27 mfma mode {mode}: rel-L2 vs rocFFT FP64 {err:.3e}", flush=True)
+        assert err < FP32_TOL
+    finally:
+        soi.free_soi_descriptor(d)

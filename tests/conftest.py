@@ -42,7 +42,8 @@ def torch_cuda():
 
 def pytest_terminal_summary(terminalreporter):
     """Achieved relative-L2 errors of the GPU parity comparisons (the largest per test, in
-    increasing order), printed and written to gpurun_out/parity_errors.json."""
+    increasing order), printed and written as parity_errors.json into the run-output folder at
+    the repository root (SOI_PARITY_OUT overrides it)."""
     gu = sys.modules.get("gpu_util")
     rec = gu.recorded() if gu else {}
     if not rec:
@@ -54,7 +55,7 @@ def pytest_terminal_summary(terminalreporter):
     for e, k, n in rows:
         each = "  [" + ", ".join(f"{v:.2e}" for v in rec[k]) + "]" if 1 < n <= 4 else f"  ({n})"
         terminalreporter.write_line(f"  {e:.2e}  {k.split('::', 1)[-1]}{each}")
-    out = os.path.join(ROOT, "gpurun_out")
+    out = os.environ.get("SOI_PARITY_OUT", os.path.join(ROOT, "gpurun_out"))
     try:
         os.makedirs(out, exist_ok=True)
         with open(os.path.join(out, "parity_errors.json"), "w") as f:

@@ -175,3 +175,36 @@ def test_batch_create_keeps_timing(soi, torch_cuda):
         if batch is not None:
             batch.close()
         soi.free_soi_descriptor(d)
+
+
+def test_batch_long_rows_match_eager(soi, torch_cuda):
+    """Members whose segment FFTs use the sequence-fastest row pass (M_hat = 1280 x 1024 at
+    N = 2^22, P = 4 and 2560 x 2048 at N = 2^24, P = 4), two of each shape in one graph: every
+    member equal to its eager compute_soi bit for bit."""
+    torch = torch_cuda
+    shapes = [(2 ** 22, 4), (2 ** 24, 4), (2 ** 22, 4), (2 ** 24, 4)]
+    print(f"case batch long rows {shapes}", flush=True)
+    descs, bufs, inputs = [], [], []
+    batch = None
+    try:
+        for i, (N, P) in enumerate(shapes):
+            descs.append(soi.make_descriptor(N, P))
+            x = torch.empty(N, dtype=torch.complex128, device="cuda")
+            assert soi.lib().soi_fill_uniform(x.data_ptr(), N, 900 + i, None) == 0
+            inputs.append(x.clone())
+            bufs.append(x)
+        torch.cuda.synchronize()
+        batch = soi.Batch(descs, [b.data_ptr() for b in bufs], autotune=False)
+        batch.run()
+        torch.cuda.synchronize()
+        for i, (d, x0) in enumerate(zip(descs, inputs)):
+            y = x0.clone()
+            torch.cuda.synchronize()
+            soi.compute_soi(d, y.data_ptr())
+            torch.cuda.synchronize()
+            assert torch.equal(bufs[i], y), f"member {i} {shapes[i]}"
+    finally:
+        if batch is not None:
+            batch.close()
+        for d in descs:
+            soi.free_soi_descriptor(d)

@@ -645,6 +645,25 @@ __global__ void k_filter_mfma_table(const double *__restrict__ taps, unsigned sh
   }
 }
 
+// Blocks per transform of a complex128 MFMA filter launch.  One transform: 256 (one per CU, each
+// walking a contiguous range of tiles).  A batched launch (blockIdx.y = member) shares about two
+// rounds of blocks over all its members, so that each block's prologue (its B operands: up to
+// 160 KB of tap table at S = 16) is spent over many tiles also when the members are small.
+// Every whole tile goes through the same loop body, so a member's results do not depend on how
+// its tiles are split over blocks (the complex64 kernels write each block's last tile through a
+// separate code copy and keep the single-transform split, so that batched results stay
+// bit-identical to eager ones).  SOI_FILTER_BATCH_BLOCKS sets the total (0: 256 per member).
+static long mfma_blocks_per_member(const SoiFilterArgs *a)
+{
+  static const long total = [] {
+    const char *e = getenv("SOI_FILTER_BATCH_BLOCKS");
+    return e ? atol(e) : 512L;
+  }();
+  if (a->b.n <= 1 || total <= 0) return 256;
+  const long nb = (total + a->b.n - 1) / a->b.n;
+  return nb < 1 ? 1 : nb;
+}
+
 template <int S> static int launch_mfma(const SoiFilterArgs *a, hipStream_t st)
 {
   typedef MfmaCfg<S> CF;
@@ -1363,7 +1382,7 @@ template <int S, int LPW> static int launch_mfma64(const SoiFilterArgs *a, hipSt
   static_assert(CF::LDS_BYTES <= 160 * 1024, "complex128 MFMA filter exceeds LDS");
   if ((long)CF::RC * CF::CH > a->M * S) return (int)hipErrorInvalidValue;  // wraps at most once
   const long ntiles = (a->nJ + CF::TJ - 1) / CF::TJ;
-  long nb = 256;
+  long nb = mfma_blocks_per_member(a);
   if (nb > ntiles) nb = ntiles;
   const int tpb = (int)((ntiles + nb - 1) / nb);
   nb = (ntiles + tpb - 1) / tpb;
